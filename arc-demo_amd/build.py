@@ -1,0 +1,70 @@
+"""Builds libarcsim.so for gfx950 with hipcc, in tree (arc-demo_amd/arc/).
+
+Every source is compiled with -ffp-contract=off (one rounding per operation,
+as the engine's parity with the reference requires) and packed-FP32
+instructions disabled.  Usage: python arc-demo_amd/build.py [--jobs N]
+"""
+
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "arc" / "libarcsim.so"
+OBJ = PKG / "_build"
+INCLUDE = PKG.parent / "include"
+
+ARCH = os.environ.get("ARC_OFFLOAD_ARCH", "gfx950")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", f"-I{INCLUDE}", f"-I{CSRC}",
+          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-fno-gpu-rdc"]
+
+HIP_SOURCES = ["kernels.hip"]
+HOST_SOURCES = ["sim.cpp", "model.cpp", "setup.cpp"]
+
+
+def hipcc() -> str:
+    path = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(path).exists():
+        raise RuntimeError("hipcc not found: the HIP toolchain is required to build libarcsim")
+    return path
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"compile failed: {cmd[-3] if len(cmd) > 3 else cmd}")
+
+
+def compile_one(src: str, device: bool) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    obj = OBJ / (Path(src).stem + ".o")
+    flags = COMMON + (DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"])
+    _run([hipcc()] + flags + ["-c", str(CSRC / src), "-o", str(obj)])
+    return obj
+
+
+def build(jobs: int = 4) -> Path:
+    tasks = [(s, True) for s in HIP_SOURCES] + [(s, False) for s in HOST_SOURCES]
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda t: compile_one(*t), tasks))
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", str(tmp)] + [str(o) for o in objs])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    a = ap.parse_args()
+    print(build(a.jobs))

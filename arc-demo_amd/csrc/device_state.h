@@ -1,0 +1,86 @@
+/*
+ * device_state.h -- HBM layout of a simulation's worlds and the launchers of
+ * the kernels that work on it.
+ *
+ * Dynamic state is structure-of-arrays with the world index fastest
+ * (field f of world i at f * stride + i), so a wavefront of 64 lanes handling
+ * 64 consecutive worlds reads and writes 256-byte runs.  Terrain is stored
+ * world-major (one world's grid contiguous), since each world only touches
+ * the few columns under its robot.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "lane_engine.h"
+
+namespace xpbd {
+
+/* LDS sizing of the reset kernel: grid up to 64 x 64 columns; value-noise
+ * lattices of (64/8+2)^2 and (64/3+2)^2 points */
+constexpr int GRID_LIMIT_CELLS = 64 * 64;
+constexpr int LAT_LIMIT = (64 / 8 + 2) * (64 / 8 + 2) + (64 / 3 + 2) * (64 / 3 + 2);
+
+struct DevWorlds {
+    int n;            /* worlds */
+    size_t stride;    /* SoA stride (n rounded up to a multiple of 64) */
+    int cells;        /* nx * ny terrain columns per world */
+    float *x;         /* [NB*3][stride] body position */
+    float *q;         /* [NB*4][stride] body orientation w,x,y,z */
+    float *v;         /* [NB*3][stride] linear velocity */
+    float *w;         /* [NB*3][stride] angular velocity */
+    float *target;    /* [NJ][stride] drive targets (clamped) */
+    float *jout;      /* [3*NJ][stride] joint angle, rate, torque */
+    uint8_t *touch;   /* [NS][stride] substeps in contact during the last step */
+    uint32_t *anchored; /* [stride] friction-anchor bit per sphere */
+    float *anc;       /* [NS*3][stride] friction anchor */
+    float *ancn;      /* [NS*3][stride] anchor normal */
+    int *unstable;    /* [stride] */
+    float *time;      /* [stride] */
+    float *terrain;   /* [n][cells] column heights */
+    float *outside;   /* [stride] height outside the grid */
+    float *cbuf;      /* [cslots*6][stride] per-substep contact list */
+    int *err;         /* [stride] contact-list overflow bits (sticky) */
+};
+
+/* Robot data the reset needs beyond the Model. */
+struct ResetK {
+    float def[NJ], lo[NJ], hi[NJ];
+    float stand, footprint;
+    int grid;         /* generated grid size n (n x n columns) */
+    float cell;
+};
+
+/* Template state broadcast at creation (the robot as built, flat ground). */
+struct TemplateK {
+    V3 x[NB];
+    Q4 q[NB];
+    float target[NJ], angle[NJ];
+};
+
+/* Slices of worlds a persistent launch steps: worlds [start, start+count)
+ * for each range; prefix = sum of the counts before it. */
+struct Range {
+    int start, count, prefix, pad;
+};
+
+/* Kernel launchers (kernels.hip).  All are asynchronous on `stream`. */
+size_t dev_bytes_per_world(int cells, int cslots);
+hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots);
+void dev_free(DevWorlds &d);
+hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, float outside, hipStream_t s);
+/* targets: n*NJ world-major, double (host ABI) or float (device inputs) */
+hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *targets_d, const float *targets_f,
+                       const Range *ranges, int n_ranges, int total, int block, int grid_blocks, hipStream_t s);
+hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_dev, const int *ids,
+                        const uint32_t *seeds, const double *difficulty, int n_ids, hipStream_t s);
+hipError_t launch_observe(const DevWorlds &d, const Model *m_dev, double *root, double *rot, double *joints,
+                          double *contacts, int *flags, hipStream_t s);
+hipError_t launch_heights(const DevWorlds &d, const Model *m_dev, const double *offsets, int k, double *out,
+                          hipStream_t s);
+hipError_t launch_terrain_at(const DevWorlds &d, const Model *m_dev, const int *ids, const double *xy, int m,
+                             double *out, hipStream_t s);
+/* full state of world i as floats (layout of the oracle's state dump) */
+hipError_t launch_dump(const DevWorlds &d, int i, float *out, hipStream_t s);
+
+} /* namespace xpbd */

@@ -1,0 +1,444 @@
+/*
+ * kernels.hip -- HIP kernels of the batched engine for gfx950.
+ *
+ *   k_step          one control step per world, one world per lane
+ *                   (lane_engine.h); a grid-stride loop over a list of world
+ *                   ranges, so one launch serves uneven per-request batches
+ *   k_reset         regenerates terrain and re-poses listed worlds, one
+ *                   wavefront per world (csrc/arc_sim.c:88-121)
+ *   k_observe / k_heights / k_terrain_at / k_dump   read-outs
+ *   k_init_template every world set to the robot as built on flat ground
+ */
+#include "device_state.h"
+#include "reset_engine.h"
+
+namespace xpbd {
+
+static inline size_t round64(size_t n) { return (n + 63) & ~(size_t)63; }
+
+/* float counts per world in the SoA block (terrain and contact list apart) */
+static const int SOA_FLOATS = NB * 3 + NB * 4 + NB * 3 + NB * 3 + NJ + 3 * NJ + NS * 3 + NS * 3 + 1 + 1;
+
+size_t dev_bytes_per_world(int cells, int cslots)
+{
+    return sizeof(float) * ((size_t)SOA_FLOATS + cells + 1 + (size_t)cslots * 6) + NS /* touch */
+           + sizeof(uint32_t) + 2 * sizeof(int);
+}
+
+hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots)
+{
+    d.n = n;
+    d.stride = round64((size_t)(n > 0 ? n : 1));
+    d.cells = cells;
+    size_t S = d.stride;
+    size_t floats = S * ((size_t)SOA_FLOATS + 1 + (size_t)cslots * 6) + (size_t)S * cells;
+    size_t bytes = floats * sizeof(float) + S * NS + S * sizeof(uint32_t) + 2 * S * sizeof(int) + 1024;
+    char *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess)
+        return e;
+    float *f = (float *)p;
+    d.x = f; f += NB * 3 * S;
+    d.q = f; f += NB * 4 * S;
+    d.v = f; f += NB * 3 * S;
+    d.w = f; f += NB * 3 * S;
+    d.target = f; f += NJ * S;
+    d.jout = f; f += 3 * NJ * S;
+    d.anc = f; f += NS * 3 * S;
+    d.ancn = f; f += NS * 3 * S;
+    d.time = f; f += S;
+    d.outside = f; f += S;
+    d.cbuf = f; f += (size_t)cslots * 6 * S;
+    d.terrain = f; f += S * cells;
+    d.unstable = (int *)f;
+    d.err = d.unstable + S;
+    d.anchored = (uint32_t *)(d.err + S);
+    d.touch = (uint8_t *)(d.anchored + S);
+    return hipSuccess;
+}
+
+void dev_free(DevWorlds &d)
+{
+    if (d.x)
+        (void)hipFree(d.x);
+    d.x = nullptr;
+}
+
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ void load_world(const DevWorlds &d, size_t i, LaneWorld &W)
+{
+    const size_t S = d.stride;
+    static_for<0, NB>([&](auto b) {
+        W.b[b].x = v3(d.x[(b * 3 + 0) * S + i], d.x[(b * 3 + 1) * S + i], d.x[(b * 3 + 2) * S + i]);
+        W.b[b].q = q4(d.q[(b * 4 + 0) * S + i], d.q[(b * 4 + 1) * S + i], d.q[(b * 4 + 2) * S + i],
+                      d.q[(b * 4 + 3) * S + i]);
+        W.b[b].v = v3(d.v[(b * 3 + 0) * S + i], d.v[(b * 3 + 1) * S + i], d.v[(b * 3 + 2) * S + i]);
+        W.b[b].w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
+    });
+    W.anchored = d.anchored[i];
+    static_for<0, NS>([&](auto s) {
+        W.anc[s] = v3(d.anc[(s * 3 + 0) * S + i], d.anc[(s * 3 + 1) * S + i], d.anc[(s * 3 + 2) * S + i]);
+        W.ancn[s] = v3(d.ancn[(s * 3 + 0) * S + i], d.ancn[(s * 3 + 1) * S + i], d.ancn[(s * 3 + 2) * S + i]);
+    });
+    W.unstable = d.unstable[i];
+    W.time = d.time[i];
+}
+
+__device__ __forceinline__ void store_world(const DevWorlds &d, size_t i, const LaneWorld &W)
+{
+    const size_t S = d.stride;
+    static_for<0, NB>([&](auto b) {
+        d.x[(b * 3 + 0) * S + i] = W.b[b].x.x; d.x[(b * 3 + 1) * S + i] = W.b[b].x.y; d.x[(b * 3 + 2) * S + i] = W.b[b].x.z;
+        d.q[(b * 4 + 0) * S + i] = W.b[b].q.w; d.q[(b * 4 + 1) * S + i] = W.b[b].q.x;
+        d.q[(b * 4 + 2) * S + i] = W.b[b].q.y; d.q[(b * 4 + 3) * S + i] = W.b[b].q.z;
+        d.v[(b * 3 + 0) * S + i] = W.b[b].v.x; d.v[(b * 3 + 1) * S + i] = W.b[b].v.y; d.v[(b * 3 + 2) * S + i] = W.b[b].v.z;
+        d.w[(b * 3 + 0) * S + i] = W.b[b].w.x; d.w[(b * 3 + 1) * S + i] = W.b[b].w.y; d.w[(b * 3 + 2) * S + i] = W.b[b].w.z;
+    });
+    static_for<0, NJ>([&](auto j) {
+        d.jout[(0 * NJ + j) * S + i] = W.angle[j];
+        d.jout[(1 * NJ + j) * S + i] = W.rate[j];
+        d.jout[(2 * NJ + j) * S + i] = W.torque[j];
+    });
+    d.anchored[i] = W.anchored;
+    static_for<0, NS>([&](auto s) {
+        d.touch[s * S + i] = (uint8_t)W.touch[s];
+        d.anc[(s * 3 + 0) * S + i] = W.anc[s].x; d.anc[(s * 3 + 1) * S + i] = W.anc[s].y; d.anc[(s * 3 + 2) * S + i] = W.anc[s].z;
+        d.ancn[(s * 3 + 0) * S + i] = W.ancn[s].x; d.ancn[(s * 3 + 1) * S + i] = W.ancn[s].y; d.ancn[(s * 3 + 2) * S + i] = W.ancn[s].z;
+    });
+    d.unstable[i] = W.unstable;
+    d.time[i] = W.time;
+}
+
+template <class TT>
+__global__ __launch_bounds__(256) void k_step(DevWorlds d, const Model *__restrict__ m, const TT *__restrict__ targets,
+                                              const Range *__restrict__ ranges, int n_ranges, int total)
+{
+    const Model &M = *m;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int i = t;
+        if (ranges) {
+            int r = 0;
+            while (r + 1 < n_ranges && ranges[r + 1].prefix <= t)
+                ++r;
+            i = ranges[r].start + (t - ranges[r].prefix);
+        }
+        const size_t S = d.stride;
+        LaneWorld W;
+        /* targets clamped to the limits, csrc/arc_sim.c:132-134 */
+        static_for<0, NJ>([&](auto j) {
+            float tg = (float)targets[(size_t)i * NJ + j];
+            W.target[j] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
+            d.target[j * S + i] = W.target[j];
+        });
+        if (d.unstable[i])
+            continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
+        load_world(d, i, W);
+        TerrainRef tr;
+        tr.h = d.terrain + (size_t)i * d.cells;
+        tr.outside = d.outside[i];
+        ContactStore cs;
+        cs.base = d.cbuf;
+        cs.stride = S;
+        cs.world = i;
+        int ov = lane_step(W, M, tr, cs);
+        store_world(d, i, W);
+        if (ov)
+            d.err[i] |= ov;
+    }
+}
+
+hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *targets_d, const float *targets_f,
+                       const Range *ranges, int n_ranges, int total, int block, int grid_blocks, hipStream_t s)
+{
+    if (total <= 0)
+        return hipSuccess;
+    if (block <= 0)
+        block = 64;
+    if (grid_blocks <= 0)
+        grid_blocks = (total + block - 1) / block;
+    if (targets_d)
+        hipLaunchKernelGGL(k_step<double>, dim3(grid_blocks), dim3(block), 0, s, d, m_dev, targets_d, ranges, n_ranges, total);
+    else
+        hipLaunchKernelGGL(k_step<float>, dim3(grid_blocks), dim3(block), 0, s, d, m_dev, targets_f, ranges, n_ranges, total);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+/* reset: terrain generation, joint noise, yaw, ground scan, FK          */
+
+__global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restrict__ m, const ResetK *__restrict__ rk,
+                                              const int *__restrict__ ids, const uint32_t *__restrict__ seeds,
+                                              const double *__restrict__ diff, int n_ids)
+{
+    __shared__ float hbuf[GRID_LIMIT_CELLS];
+    __shared__ float lat[LAT_LIMIT];
+    __shared__ float draws[NJ + 1];
+    const Model &M = *m;
+    const ResetK &R = *rk;
+    const int k = blockIdx.x, tid = threadIdx.x;
+    if (k >= n_ids)
+        return;
+    const int i = ids[k];
+    const float dfc = (float)diff[k];
+    const int n = R.grid, cells = n * n;
+    const bool gen = dfc > 0.0f;
+    const size_t S = d.stride;
+
+    /* the draws are one sequential stream: lane 0 produces them */
+    if (tid == 0)
+        reset_draws(seeds[k], gen, n, lat, hbuf, draws);
+    __syncthreads();
+    for (int c = tid; c < cells; c += 64)
+        hbuf[c] = gen ? column_height(lat, n, dfc, hbuf[c], c) : 0.0f;
+    __syncthreads();
+    TerrainRef tl;
+    tl.h = hbuf;
+    tl.outside = -100.0f;
+    if (gen) { /* shift so that the column at the origin is at z = 0 */
+        float base = terrain_height(M.terr, tl, 0.0f, 0.0f);
+        __syncthreads();
+        for (int c = tid; c < cells; c += 64)
+            hbuf[c] -= base;
+        __syncthreads();
+    }
+    for (int c = tid; c < cells; c += 64)
+        d.terrain[(size_t)i * d.cells + c] = hbuf[c];
+
+    if (tid == 0) {
+        float ang[NJ];
+        for (int j = 0; j < NJ; ++j)
+            ang[j] = clampf_(R.def[j] + draws[j], R.lo[j], R.hi[j]);
+        float ground = ground_under(M.terr, tl, R.footprint, R.cell);
+        V3 bx[NB];
+        Q4 bq[NB];
+        pose_bodies(M, v3(0.0f, 0.0f, ground + R.stand + 0.03f), qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]), ang, bx, bq);
+        for (int b = 0; b < NB; ++b) {
+            d.x[(b * 3 + 0) * S + i] = bx[b].x; d.x[(b * 3 + 1) * S + i] = bx[b].y; d.x[(b * 3 + 2) * S + i] = bx[b].z;
+            d.q[(b * 4 + 0) * S + i] = bq[b].w; d.q[(b * 4 + 1) * S + i] = bq[b].x;
+            d.q[(b * 4 + 2) * S + i] = bq[b].y; d.q[(b * 4 + 3) * S + i] = bq[b].z;
+            for (int c = 0; c < 3; ++c) {
+                d.v[(b * 3 + c) * S + i] = 0.0f;
+                d.w[(b * 3 + c) * S + i] = 0.0f;
+            }
+        }
+        for (int j = 0; j < NJ; ++j) {
+            d.target[j * S + i] = ang[j];
+            d.jout[(0 * NJ + j) * S + i] = ang[j];
+            d.jout[(1 * NJ + j) * S + i] = 0.0f;
+            d.jout[(2 * NJ + j) * S + i] = 0.0f;
+        }
+        for (int s = 0; s < NS; ++s) {
+            d.touch[s * S + i] = 0;
+            for (int c = 0; c < 3; ++c) {
+                d.anc[(s * 3 + c) * S + i] = 0.0f;
+                d.ancn[(s * 3 + c) * S + i] = 0.0f;
+            }
+        }
+        d.anchored[i] = 0;
+        d.unstable[i] = 0;
+        d.time[i] = 0.0f;
+        d.outside[i] = -100.0f;
+        d.err[i] = 0;
+    }
+}
+
+hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_dev, const int *ids,
+                        const uint32_t *seeds, const double *difficulty, int n_ids, hipStream_t s)
+{
+    if (n_ids <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_reset, dim3(n_ids), dim3(64), 0, s, d, m_dev, r_dev, ids, seeds, difficulty, n_ids);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+
+__global__ void k_init_template(DevWorlds d, const TemplateK *__restrict__ tk, float outside)
+{
+    const size_t S = d.stride;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)d.n; i += (size_t)gridDim.x * blockDim.x) {
+        for (int b = 0; b < NB; ++b) {
+            d.x[(b * 3 + 0) * S + i] = tk->x[b].x; d.x[(b * 3 + 1) * S + i] = tk->x[b].y; d.x[(b * 3 + 2) * S + i] = tk->x[b].z;
+            d.q[(b * 4 + 0) * S + i] = tk->q[b].w; d.q[(b * 4 + 1) * S + i] = tk->q[b].x;
+            d.q[(b * 4 + 2) * S + i] = tk->q[b].y; d.q[(b * 4 + 3) * S + i] = tk->q[b].z;
+            for (int c = 0; c < 3; ++c) {
+                d.v[(b * 3 + c) * S + i] = 0.0f;
+                d.w[(b * 3 + c) * S + i] = 0.0f;
+            }
+        }
+        for (int j = 0; j < NJ; ++j) {
+            d.target[j * S + i] = tk->target[j];
+            d.jout[(0 * NJ + j) * S + i] = tk->angle[j];
+            d.jout[(1 * NJ + j) * S + i] = 0.0f;
+            d.jout[(2 * NJ + j) * S + i] = 0.0f;
+        }
+        for (int s = 0; s < NS; ++s) {
+            d.touch[s * S + i] = 0;
+            for (int c = 0; c < 3; ++c) {
+                d.anc[(s * 3 + c) * S + i] = 0.0f;
+                d.ancn[(s * 3 + c) * S + i] = 0.0f;
+            }
+        }
+        d.anchored[i] = 0;
+        d.unstable[i] = 0;
+        d.time[i] = 0.0f;
+        d.outside[i] = outside;
+        d.err[i] = 0;
+        for (int c = 0; c < d.cells; ++c)
+            d.terrain[i * d.cells + c] = 0.0f;
+    }
+}
+
+hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, float outside, hipStream_t s)
+{
+    int blocks = (d.n + 255) / 256;
+    if (blocks < 1)
+        blocks = 1;
+    hipLaunchKernelGGL(k_init_template, dim3(blocks), dim3(256), 0, s, d, tk_dev, outside);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+/* read-outs, csrc/arc_sim.c:140-210                                    */
+
+__global__ void k_observe(DevWorlds d, const Model *__restrict__ m, double *root, double *rot, double *joints,
+                          double *contacts, int *flags)
+{
+    const Model &M = *m;
+    const size_t S = d.stride;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)d.n)
+        return;
+    Q4 q = q4(d.q[0 * S + i], d.q[1 * S + i], d.q[2 * S + i], d.q[3 * S + i]);
+    if (root) {
+        double *r = root + i * 13;
+        r[0] = d.x[0 * S + i]; r[1] = d.x[1 * S + i]; r[2] = d.x[2 * S + i];
+        r[3] = q.w; r[4] = q.x; r[5] = q.y; r[6] = q.z;
+        r[7] = d.v[0 * S + i]; r[8] = d.v[1 * S + i]; r[9] = d.v[2 * S + i];
+        r[10] = d.w[0 * S + i]; r[11] = d.w[1 * S + i]; r[12] = d.w[2 * S + i];
+    }
+    if (rot) {
+        M3 R;
+        m_from_q(R, q);
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c)
+                rot[i * 9 + a * 3 + c] = R.e[a][c];
+    }
+    if (joints) {
+        for (int j = 0; j < 3 * NJ; ++j)
+            joints[i * 36 + j] = d.jout[j * S + i];
+    }
+    if (contacts) {
+        float o[NCH];
+        for (int c = 0; c < NCH; ++c)
+            o[c] = 0.0f;
+        static_for<0, NS>([&](auto s) {
+            constexpr int ch = sphere_channel(s);
+            o[ch] = maxf_(o[ch], (float)d.touch[s * S + i] * M.inv_sub);
+        });
+        for (int c = 0; c < NCH; ++c)
+            contacts[i * NCH + c] = o[c];
+    }
+    if (flags)
+        flags[i] = d.unstable[i];
+}
+
+hipError_t launch_observe(const DevWorlds &d, const Model *m_dev, double *root, double *rot, double *joints,
+                          double *contacts, int *flags, hipStream_t s)
+{
+    if (d.n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_observe, dim3((d.n + 255) / 256), dim3(256), 0, s, d, m_dev, root, rot, joints, contacts, flags);
+    return hipGetLastError();
+}
+
+__global__ void k_heights(DevWorlds d, const Model *__restrict__ m, const double *__restrict__ offsets, int k, double *out)
+{
+    const Model &M = *m;
+    const size_t S = d.stride;
+    size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)d.n * k)
+        return;
+    size_t i = t / k;
+    int j = (int)(t % k);
+    Q4 q = q4(d.q[0 * S + i], d.q[1 * S + i], d.q[2 * S + i], d.q[3 * S + i]);
+    V3 fwd = qrot(q, v3(1.0f, 0.0f, 0.0f));
+    float yaw = f_atan2(fwd.y, fwd.x), c = f_cos(yaw), sn = f_sin(yaw);
+    float f = (float)offsets[2 * j], l = (float)offsets[2 * j + 1];
+    float px = d.x[0 * S + i], py = d.x[1 * S + i];
+    TerrainRef tr;
+    tr.h = d.terrain + i * d.cells;
+    tr.outside = d.outside[i];
+    out[t] = terrain_height(M.terr, tr, px + c * f - sn * l, py + sn * f + c * l);
+}
+
+hipError_t launch_heights(const DevWorlds &d, const Model *m_dev, const double *offsets, int k, double *out,
+                          hipStream_t s)
+{
+    size_t total = (size_t)d.n * k;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_heights, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d, m_dev, offsets, k, out);
+    return hipGetLastError();
+}
+
+__global__ void k_terrain_at(DevWorlds d, const Model *__restrict__ m, const int *__restrict__ ids,
+                             const double *__restrict__ xy, int mcount, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mcount)
+        return;
+    int i = ids[t];
+    TerrainRef tr;
+    tr.h = d.terrain + (size_t)i * d.cells;
+    tr.outside = d.outside[i];
+    out[t] = terrain_height(m->terr, tr, (float)xy[2 * t], (float)xy[2 * t + 1]);
+}
+
+hipError_t launch_terrain_at(const DevWorlds &d, const Model *m_dev, const int *ids, const double *xy, int mcount,
+                             double *out, hipStream_t s)
+{
+    if (mcount <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_terrain_at, dim3((mcount + 255) / 256), dim3(256), 0, s, d, m_dev, ids, xy, mcount, out);
+    return hipGetLastError();
+}
+
+/* world i in the oracle's dump layout: bodies (x3 q4 v3 w3), joints (target
+ * angle rate torque), spheres (touch anchored anchor3 anchor_n3), unstable, time */
+__global__ void k_dump(DevWorlds d, int i, float *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const size_t S = d.stride;
+    int o = 0;
+    for (int b = 0; b < NB; ++b) {
+        for (int c = 0; c < 3; ++c) out[o++] = d.x[(b * 3 + c) * S + i];
+        for (int c = 0; c < 4; ++c) out[o++] = d.q[(b * 4 + c) * S + i];
+        for (int c = 0; c < 3; ++c) out[o++] = d.v[(b * 3 + c) * S + i];
+        for (int c = 0; c < 3; ++c) out[o++] = d.w[(b * 3 + c) * S + i];
+    }
+    for (int j = 0; j < NJ; ++j) {
+        out[o++] = d.target[j * S + i];
+        out[o++] = d.jout[(0 * NJ + j) * S + i];
+        out[o++] = d.jout[(1 * NJ + j) * S + i];
+        out[o++] = d.jout[(2 * NJ + j) * S + i];
+    }
+    for (int s = 0; s < NS; ++s) {
+        out[o++] = (float)d.touch[s * S + i];
+        out[o++] = (float)((d.anchored[i] >> s) & 1u);
+        for (int c = 0; c < 3; ++c) out[o++] = d.anc[(s * 3 + c) * S + i];
+        for (int c = 0; c < 3; ++c) out[o++] = d.ancn[(s * 3 + c) * S + i];
+    }
+    out[o++] = (float)d.unstable[i];
+    out[o++] = d.time[i];
+}
+
+hipError_t launch_dump(const DevWorlds &d, int i, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dump, dim3(1), dim3(64), 0, s, d, i, out);
+    return hipGetLastError();
+}
+
+} /* namespace xpbd */

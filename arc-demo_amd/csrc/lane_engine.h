@@ -1,0 +1,173 @@
+/*
+ * lane_engine.h -- one control step of one quadruped world, run by a single
+ * lane with the whole world state in registers.
+ *
+ * The robot's topology is fixed (csrc/arc_robot.c:180-229): 13 bodies
+ * (torso, then coxa / femur / tibia per leg), 12 hinges (joint 3l+k drives
+ * body 3l+k+1), 21 spheres (torso; per leg a femur sphere and four tibia
+ * spheres).  Every loop over bodies, joints and spheres is unrolled at compile
+ * time, so all indices are constants and the solver runs in exactly the
+ * reference's Gauss-Seidel order (csrc/arc_phys.c:707-747) with each world in
+ * its own lane.  Numeric constants come from the Model (device constant
+ * memory in the kernels).
+ *
+ * The same code compiles for the host: the CPU parity harness of the tests
+ * runs it against the oracle without a GPU.
+ */
+#pragma once
+#include <type_traits>
+#include "xpbd_core.h"
+
+namespace xpbd {
+
+constexpr int NB = 13, NJ = 12, NS = 21, LEGS = 4;
+constexpr int NCH = LEGS + 2;          /* contact channels: feet, torso, other */
+constexpr int MAXC = 128;              /* contact record cap, arc_phys.h:22 */
+
+constexpr int body_kind(int b) { return (b == 0 || (b - 1) % 3 == 0) ? KISO : KANI; }
+constexpr int joint_parent(int j) { return j % 3 == 0 ? 0 : j; }
+constexpr int joint_child(int j) { return j + 1; }
+constexpr int sphere_body(int s) { return s == 0 ? 0 : ((s - 1) % 5 == 0 ? 2 + 3 * ((s - 1) / 5) : 3 + 3 * ((s - 1) / 5)); }
+constexpr int sphere_at(int s) { return (s == 0 || (s - 1) % 5 == 0) ? AT_ZERO : AT_Z; }
+constexpr int sphere_channel(int s) { return s == 0 ? LEGS : ((s - 1) % 5 == 4 ? (s - 1) / 5 : LEGS + 1); }
+/* bodies carrying spheres, in contact-list slot order */
+constexpr int NCB = 9;
+constexpr int cbody(int i) { return i == 0 ? 0 : (i <= 4 ? 2 + 3 * (i - 1) : 3 + 3 * (i - 5)); }
+constexpr int cbody_index(int b) { return b == 0 ? 0 : ((b - 2) % 3 == 0 ? 1 + (b - 2) / 3 : 5 + (b - 3) / 3); }
+
+template <int I, int N, class F> XD void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+struct Model {
+    BodyK body[NB];
+    JointK joint[NJ];
+    SphereK sphere[NS];
+    TerrainK terr;
+    V3 g;
+    float mus, mud;
+    float dt, h, hh, inv_h, two_h, inv_sub;
+    int substeps;
+    /* contact list: per sphere-carrying body a slot range of the per-world
+     * scratch list, sized from the spheres' cell windows */
+    int cslot[NCB], ccap[NCB];
+    int cslots;
+};
+
+/* Register-resident state of one world during a step. */
+struct LaneWorld {
+    Body b[NB];
+    float target[NJ], tacc[NJ];
+    float angle[NJ], rate[NJ], torque[NJ];
+    int touch[NS];
+    uint32_t anchored;    /* bit s: sphere s has a valid friction anchor */
+    V3 anc[NS], ancn[NS];
+    int unstable;
+    float time;
+};
+
+/* Contact list storage: slot k field f of this world. */
+struct ContactStore {
+    float *base;   /* [slot * 6 + field] * stride + world */
+    size_t stride;
+    size_t world;
+    XD void put(int slot, V3 l, V3 n) const
+    {
+        float *p = base + (size_t)slot * 6 * stride + world;
+        p[0] = l.x; p[stride] = l.y; p[2 * stride] = l.z;
+        p[3 * stride] = n.x; p[4 * stride] = n.y; p[5 * stride] = n.z;
+    }
+    XD void get(int slot, V3 &l, V3 &n) const
+    {
+        const float *p = base + (size_t)slot * 6 * stride + world;
+        l = v3(p[0], p[stride], p[2 * stride]);
+        n = v3(p[3 * stride], p[4 * stride], p[5 * stride]);
+    }
+};
+
+/* One control step (arc_world_step, csrc/arc_phys.c:707-747).
+ * Returns a bit mask of bodies whose contact list overflowed its slots
+ * (never expected for the robot; reported as an error by the host). */
+XD int lane_step(LaneWorld &W, const Model &M, const TerrainRef &tr, const ContactStore &cs)
+{
+    int overflow = 0;
+    /* prologue: refresh caches, clear counters (lim is precomputed per joint) */
+    static_for<0, NB>([&](auto bi) { bsync<body_kind(bi)>(W.b[bi], M.body[bi]); });
+    static_for<0, NS>([&](auto si) { W.touch[si] = 0; });
+    static_for<0, NJ>([&](auto ji) { W.tacc[ji] = 0.0f; });
+
+    for (int sub = 0; sub < M.substeps; ++sub) {
+        static_for<0, NB>([&](auto bi) { integrate<body_kind(bi)>(W.b[bi], M.body[bi], M.g, M.h); });
+
+        /* joint positions, Gauss-Seidel in joint order */
+        static_for<0, NJ>([&](auto ji) {
+            constexpr int P = joint_parent(ji), C = joint_child(ji);
+            joint_positions<body_kind(P), body_kind(C)>(&W.b[P], &M.body[P], W.b[C], M.body[C], M.joint[ji],
+                                                        W.target[ji], M.hh, W.tacc[ji]);
+        });
+
+        /* spheres against the terrain, in sphere order; contacts recorded
+         * per body (the velocity pass below only couples a contact with its
+         * own body, so per-body order is the reference's order) */
+        int ncount[NCB];
+        int total = 0;
+        static_for<0, NCB>([&](auto ci) { ncount[ci] = 0; });
+        static_for<0, NS>([&](auto si) {
+            constexpr int B = sphere_body(si);
+            constexpr int CI = cbody_index(B);
+            int anch = (W.anchored >> si) & 1u;
+            auto rec = [&](V3 l, V3 n) {
+                if (total < MAXC) {
+                    if (ncount[CI] < M.ccap[CI])
+                        cs.put(M.cslot[CI] + ncount[CI], l, n);
+                    else
+                        overflow |= 1 << CI;
+                    ncount[CI]++;
+                    total++;
+                }
+            };
+            collide<body_kind(B), sphere_at(si)>(W.b[B], M.body[B], M.sphere[si].at, M.sphere[si].rad,
+                                                 W.touch[si], anch, W.anc[si], W.ancn[si], M.terr, tr, rec,
+                                                 M.mus, M.mud);
+            W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
+        });
+
+        static_for<0, NB>([&](auto bi) { velocities(W.b[bi], M.inv_h, M.two_h); });
+
+        static_for<0, NJ>([&](auto ji) {
+            constexpr int P = joint_parent(ji), C = joint_child(ji);
+            joint_velocities<body_kind(P), body_kind(C)>(&W.b[P], &M.body[P], W.b[C], M.body[C], M.joint[ji],
+                                                         M.h, W.tacc[ji]);
+        });
+
+        static_for<0, NCB>([&](auto ci) {
+            constexpr int B = cbody(ci);
+            int n = ncount[ci] < M.ccap[ci] ? ncount[ci] : M.ccap[ci];
+            for (int k = 0; k < n; ++k) {
+                V3 l, nn;
+                cs.get(M.cslot[ci] + k, l, nn);
+                contact_velocity<body_kind(B)>(W.b[B], M.body[B], l, nn);
+            }
+        });
+    }
+
+    /* epilogue: joint outputs, time, stability */
+    static_for<0, NJ>([&](auto ji) {
+        constexpr int P = joint_parent(ji), C = joint_child(ji);
+        W.angle[ji] = joint_angle<true>(&W.b[P], W.b[C], M.joint[ji]);
+        W.rate[ji] = joint_rate<true>(&W.b[P], W.b[C], M.joint[ji]);
+        W.torque[ji] = W.tacc[ji] / (float)M.substeps;
+    });
+    W.time += M.dt;
+    bool bad = false;
+    static_for<0, NB>([&](auto bi) { bad |= body_unstable(W.b[bi]); });
+    if (bad)
+        W.unstable = 1;
+    return overflow;
+}
+
+} /* namespace xpbd */

@@ -1,0 +1,65 @@
+/*
+ * arc_gpu.h -- additive API of libarcsim beside the reference's arc_sim.h:
+ * error reporting, device-resident stepping, the task layer's fall rules with
+ * auto-reset on the device, launch tuning and state read-back for tests.
+ * None of these has a counterpart in the reference's C API; they exist for
+ * callers that keep their inputs on the GPU (benchmarks, the step server,
+ * PyTorch policies) and for the parity tests.
+ */
+#ifndef ARC_GPU_H
+#define ARC_GPU_H
+
+#include "arc_sim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sticky error state of a handle: 0 = none.  arcx_error_message returns a
+ * static description of the first error since the last clear. */
+int arcx_last_error(const arc_sim *s);
+const char *arcx_error_message(const arc_sim *s);
+void arcx_clear_error(arc_sim *s);
+
+/* Number of HIP devices visible (0 when the library cannot run). */
+int arcx_device_count(void);
+
+/* n successive outputs of the engine's xorshift128 generator seeded `seed`
+ * (the stream the reset draws from; also used to make synthetic inputs). */
+void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out);
+
+/* Size of the per-world flat state of arcx_state (same layout as the
+ * oracle's dump): 13 bodies x (x3 q4 v3 w3), 12 joints x (target angle rate
+ * torque), 21 spheres x (touch anchored anchor3 anchor_n3), unstable, time. */
+int arcx_state_size(void);
+/* State of world i (i = -1: the template) as float64. */
+int arcx_state(const arc_sim *s, int i, double *out);
+/* Terrain of world i: grid*grid heights then the outside height. */
+int arcx_terrain(const arc_sim *s, int i, double *out);
+
+/* Device-resident stepping: targets_dev is an N x 12 float32 device array
+ * (world-major).  Asynchronous on the handle's stream; arcx_sync waits. */
+int arcx_step_device(arc_sim *s, const float *targets_dev);
+int arcx_sync(arc_sim *s);
+
+/* Benchmark helper: `steps` control steps back to back, step k reading
+ * targets_dev + k * step_floats; kernel_ms = summed HIP-event time of the
+ * step kernels, wall_ms = event time from before the first to after the last. */
+int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long step_floats, double *kernel_ms,
+                    double *wall_ms);
+
+/* Launch shape of the step kernel: threads per block (a multiple of 64) and
+ * blocks (0 = one pass over the worlds). */
+int arcx_set_launch(arc_sim *s, int block, int grid_blocks);
+
+/* HBM bytes the library holds per world. */
+double arcx_bytes_per_world(const arc_sim *s);
+
+/* Contact-list overflow flags of world i (0 expected). */
+int arcx_world_error(const arc_sim *s, int i);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

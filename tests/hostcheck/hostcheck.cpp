@@ -1,0 +1,158 @@
+/*
+ * hostcheck.cpp -- test harness: the device engine's own code (lane_engine.h,
+ * reset_engine.h, setup.cpp, model.cpp) compiled for the host CPU and driven
+ * like the arc_sim API, so the parity tests can compare it with the oracle
+ * without a GPU.  The only difference from the device build is the libm used
+ * for the binary64 atan2 / sin / cos (glibc here, the device library there).
+ * Never part of the product.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "reset_engine.h"
+#include "setup.h"
+
+using namespace xpbd;
+
+struct HC {
+    int n;
+    SimSetup su;
+    std::vector<LaneWorld> w;
+    std::vector<float> terrain, outside;
+    std::vector<float> cbuf;
+    std::vector<int> err;
+};
+
+extern "C" {
+
+HC *hc_create(int n, double dt, int substeps, int grid, double cell)
+{
+    HC *h = new HC();
+    h->n = n;
+    if (!sim_setup(h->su, dt, substeps, grid, cell)) {
+        delete h;
+        return nullptr;
+    }
+    const int cells = h->su.grid * h->su.grid;
+    h->w.resize(n);
+    h->terrain.assign((size_t)n * cells, 0.0f);
+    h->outside.assign(n, h->su.tmpl.outside);
+    h->cbuf.assign((size_t)h->su.model.cslots * 6, 0.0f);
+    h->err.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        LaneWorld &W = h->w[i];
+        memset(&W, 0, sizeof(W));
+        for (int b = 0; b < NB; ++b) {
+            W.b[b].x = h->su.tk.x[b];
+            W.b[b].q = h->su.tk.q[b];
+        }
+        for (int j = 0; j < NJ; ++j) {
+            W.target[j] = h->su.tk.target[j];
+            W.angle[j] = h->su.tk.angle[j];
+        }
+    }
+    return h;
+}
+
+void hc_destroy(HC *h) { delete h; }
+
+void hc_reset(HC *h, const int *ids, int n_ids, const unsigned *seeds, const double *diff)
+{
+    const Model &M = h->su.model;
+    const ResetK &R = h->su.rk;
+    const int n = R.grid, cells = n * n;
+    std::vector<float> lat(LAT_LIMIT), hb(cells);
+    float draws[NJ + 1];
+    for (int k = 0; k < n_ids; ++k) {
+        int i = ids[k];
+        float d = (float)diff[k];
+        bool gen = d > 0.0f;
+        reset_draws(seeds[k], gen, n, lat.data(), hb.data(), draws);
+        for (int c = 0; c < cells; ++c)
+            hb[c] = gen ? column_height(lat.data(), n, d, hb[c], c) : 0.0f;
+        TerrainRef tl;
+        tl.h = hb.data();
+        tl.outside = -100.0f;
+        if (gen) {
+            float base = terrain_height(M.terr, tl, 0.0f, 0.0f);
+            for (int c = 0; c < cells; ++c)
+                hb[c] -= base;
+        }
+        memcpy(&h->terrain[(size_t)i * cells], hb.data(), cells * sizeof(float));
+        h->outside[i] = -100.0f;
+        float ang[NJ];
+        for (int j = 0; j < NJ; ++j)
+            ang[j] = clampf_(R.def[j] + draws[j], R.lo[j], R.hi[j]);
+        float ground = ground_under(M.terr, tl, R.footprint, R.cell);
+        V3 bx[NB];
+        Q4 bq[NB];
+        pose_bodies(M, v3(0.0f, 0.0f, ground + R.stand + 0.03f), qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]), ang, bx, bq);
+        LaneWorld &W = h->w[i];
+        memset(&W, 0, sizeof(W));
+        for (int b = 0; b < NB; ++b) {
+            W.b[b].x = bx[b];
+            W.b[b].q = bq[b];
+        }
+        for (int j = 0; j < NJ; ++j) {
+            W.target[j] = ang[j];
+            W.angle[j] = ang[j];
+        }
+        h->err[i] = 0;
+    }
+}
+
+void hc_step(HC *h, const double *targets)
+{
+    const Model &M = h->su.model;
+    const int cells = h->su.grid * h->su.grid;
+    for (int i = 0; i < h->n; ++i) {
+        LaneWorld &W = h->w[i];
+        for (int j = 0; j < NJ; ++j)
+            W.target[j] = clampf_((float)targets[i * NJ + j], M.joint[j].lo, M.joint[j].hi);
+        if (W.unstable)
+            continue;
+        TerrainRef tr;
+        tr.h = &h->terrain[(size_t)i * cells];
+        tr.outside = h->outside[i];
+        ContactStore cs;
+        cs.base = h->cbuf.data();
+        cs.stride = 1;
+        cs.world = 0;
+        h->err[i] |= lane_step(W, M, tr, cs);
+    }
+}
+
+/* oracle dump layout */
+void hc_state(const HC *h, int i, double *out)
+{
+    const LaneWorld &W = h->w[i];
+    int o = 0;
+    for (int b = 0; b < NB; ++b) {
+        const Body &B = W.b[b];
+        double v[13] = { B.x.x, B.x.y, B.x.z, B.q.w, B.q.x, B.q.y, B.q.z, B.v.x, B.v.y, B.v.z, B.w.x, B.w.y, B.w.z };
+        for (int c = 0; c < 13; ++c) out[o++] = v[c];
+    }
+    for (int j = 0; j < NJ; ++j) {
+        out[o++] = W.target[j]; out[o++] = W.angle[j]; out[o++] = W.rate[j]; out[o++] = W.torque[j];
+    }
+    for (int s = 0; s < NS; ++s) {
+        out[o++] = W.touch[s];
+        out[o++] = (W.anchored >> s) & 1u;
+        out[o++] = W.anc[s].x; out[o++] = W.anc[s].y; out[o++] = W.anc[s].z;
+        out[o++] = W.ancn[s].x; out[o++] = W.ancn[s].y; out[o++] = W.ancn[s].z;
+    }
+    out[o++] = W.unstable;
+    out[o++] = W.time;
+}
+
+void hc_terrain(const HC *h, int i, double *out)
+{
+    const int cells = h->su.grid * h->su.grid;
+    for (int c = 0; c < cells; ++c)
+        out[c] = h->terrain[(size_t)i * cells + c];
+    out[cells] = h->outside[i];
+}
+
+int hc_error(const HC *h, int i) { return h->err[i]; }
+
+}
