@@ -39,6 +39,7 @@ struct DevWorlds {
     float *time;      /* [stride] */
     float *terrain;   /* [n][cells] column heights */
     float *outside;   /* [stride] height outside the grid */
+    float *difficulty; /* [stride] terrain difficulty of the last reset */
     float *cbuf;      /* [cslots*6][stride] per-substep contact list */
     int *err;         /* [stride] contact-list overflow bits (sticky) */
 };
@@ -58,6 +59,26 @@ struct TemplateK {
     float target[NJ], angle[NJ];
 };
 
+/* Task layer (arc/task.py:180-203, arc/env.py:99-117): fall rules and the
+ * auto-reset of fallen worlds, kept on the device. */
+struct TaskK {
+    double half_extent;   /* 0.5 * grid * cell, from the caller's double cell */
+    double void_z;        /* torso below this height: fell into the void */
+    double flip_up_z;     /* torso up-axis z below this: flipped */
+    int beached_steps;    /* more consecutive steps of torso contact: beached */
+    int n;                /* worlds (seed schedule: seed = i + n * episode) */
+};
+struct TaskState {
+    int *torso_steps;     /* [n] consecutive steps with torso contact */
+    uint32_t *episode;    /* [n] resets so far */
+    int *cause;           /* [n] cause of the last step's termination (0 none) */
+    int *list;            /* [n] worlds to reset this step */
+    uint32_t *seeds;      /* [n] their seeds */
+    float *diff;          /* [n] their difficulties */
+    int *count;           /* [1] entries in list */
+    unsigned long long *stats; /* [5] terminations per cause (index 0: steps) */
+};
+
 /* Slices of worlds a persistent launch steps: worlds [start, start+count)
  * for each range; prefix = sum of the counts before it. */
 struct Range {
@@ -72,8 +93,17 @@ hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, flo
 /* targets: n*NJ world-major, double (host ABI) or float (device inputs) */
 hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *targets_d, const float *targets_f,
                        const Range *ranges, int n_ranges, int total, int block, int grid_blocks, hipStream_t s);
+/* Resets ids[k] with seeds[k] and difficulty (double or float array), for k <
+ * *count_dev when count_dev is given (entries decided on the device), else
+ * k < n_ids; at most `max_blocks` blocks loop over the entries. */
 hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_dev, const int *ids,
-                        const uint32_t *seeds, const double *difficulty, int n_ids, hipStream_t s);
+                        const uint32_t *seeds, const double *difficulty_d, const float *difficulty_f, int n_ids,
+                        const int *count_dev, int max_blocks, hipStream_t s);
+/* p[0..n) = v with a kernel (safe inside captured graphs, unlike memset nodes) */
+hipError_t launch_fill_int(int *p, int v, int n, hipStream_t s);
+/* Fall rules after a step; fallen worlds are appended to the task list
+ * (ts.count must be zero before). */
+hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts, hipStream_t s);
 hipError_t launch_observe(const DevWorlds &d, const Model *m_dev, double *root, double *rot, double *joints,
                           double *contacts, int *flags, hipStream_t s);
 hipError_t launch_heights(const DevWorlds &d, const Model *m_dev, const double *offsets, int k, double *out,

@@ -17,7 +17,7 @@ namespace xpbd {
 static inline size_t round64(size_t n) { return (n + 63) & ~(size_t)63; }
 
 /* float counts per world in the SoA block (terrain and contact list apart) */
-static const int SOA_FLOATS = NB * 3 + NB * 4 + NB * 3 + NB * 3 + NJ + 3 * NJ + NS * 3 + NS * 3 + 1 + 1;
+static const int SOA_FLOATS = NB * 3 + NB * 4 + NB * 3 + NB * 3 + NJ + 3 * NJ + NS * 3 + NS * 3 + 1 + 1 + 1;
 
 size_t dev_bytes_per_world(int cells, int cslots)
 {
@@ -48,6 +48,7 @@ hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots)
     d.ancn = f; f += NS * 3 * S;
     d.time = f; f += S;
     d.outside = f; f += S;
+    d.difficulty = f; f += S;
     d.cbuf = f; f += (size_t)cslots * 6 * S;
     d.terrain = f; f += S * cells;
     d.unstable = (int *)f;
@@ -169,18 +170,19 @@ hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *tar
 
 __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restrict__ m, const ResetK *__restrict__ rk,
                                               const int *__restrict__ ids, const uint32_t *__restrict__ seeds,
-                                              const double *__restrict__ diff, int n_ids)
+                                              const double *__restrict__ diff_d, const float *__restrict__ diff_f,
+                                              int n_ids, const int *__restrict__ count_dev)
 {
     __shared__ float hbuf[GRID_LIMIT_CELLS];
     __shared__ float lat[LAT_LIMIT];
     __shared__ float draws[NJ + 1];
     const Model &M = *m;
     const ResetK &R = *rk;
-    const int k = blockIdx.x, tid = threadIdx.x;
-    if (k >= n_ids)
-        return;
+    const int tid = threadIdx.x;
+    const int count = count_dev ? *count_dev : n_ids;
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
     const int i = ids[k];
-    const float dfc = (float)diff[k];
+    const float dfc = diff_d ? (float)diff_d[k] : diff_f[k];
     const int n = R.grid, cells = n * n;
     const bool gen = dfc > 0.0f;
     const size_t S = d.stride;
@@ -239,16 +241,87 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
         d.outside[i] = -100.0f;
+        d.difficulty[i] = dfc;
         d.err[i] = 0;
+    }
+    __syncthreads(); /* LDS is reused by the next entry */
     }
 }
 
 hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_dev, const int *ids,
-                        const uint32_t *seeds, const double *difficulty, int n_ids, hipStream_t s)
+                        const uint32_t *seeds, const double *difficulty_d, const float *difficulty_f, int n_ids,
+                        const int *count_dev, int max_blocks, hipStream_t s)
 {
-    if (n_ids <= 0)
+    int blocks = count_dev ? max_blocks : (n_ids < max_blocks || max_blocks <= 0 ? n_ids : max_blocks);
+    if (blocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_reset, dim3(n_ids), dim3(64), 0, s, d, m_dev, r_dev, ids, seeds, difficulty, n_ids);
+    hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(64), 0, s, d, m_dev, r_dev, ids, seeds, difficulty_d, difficulty_f,
+                       n_ids, count_dev);
+    return hipGetLastError();
+}
+
+/* fall rules of the task after a step, arc/task.py:180-203; comparisons in
+ * binary64 on the binary32 state, as the task makes them on float64 copies */
+__global__ void k_falls(DevWorlds d, const Model *__restrict__ m, TaskK tk, TaskState ts)
+{
+    const size_t S = d.stride;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n)
+        return;
+    float r[13];
+    for (int c = 0; c < 3; ++c) r[c] = d.x[c * S + i];
+    for (int c = 0; c < 4; ++c) r[3 + c] = d.q[c * S + i];
+    for (int c = 0; c < 3; ++c) r[7 + c] = d.v[c * S + i];
+    for (int c = 0; c < 3; ++c) r[10 + c] = d.w[c * S + i];
+    bool bad = d.unstable[i] != 0;
+    for (int c = 0; c < 13; ++c)
+        bad |= !isfinite(r[c]);
+    float torso = 0.0f;
+    static_for<0, NS>([&](auto s) {
+        if (sphere_channel(s) == LEGS)
+            torso = maxf_(torso, (float)d.touch[s * S + i] * m->inv_sub);
+    });
+    int steps = torso > 0.0f ? ts.torso_steps[i] + 1 : 0;
+    M3 R;
+    m_from_q(R, q4(r[3], r[4], r[5], r[6]));
+    bool vd = fabs((double)r[0]) > tk.half_extent || fabs((double)r[1]) > tk.half_extent || (double)r[2] < tk.void_z;
+    bool flip = (double)R.e[2][2] < tk.flip_up_z;
+    bool beached = steps > tk.beached_steps;
+    int cause = bad ? 1 : vd ? 2 : flip ? 3 : beached ? 4 : 0;
+    ts.cause[i] = cause;
+    if (cause) {
+        uint32_t ep = ts.episode[i] + 1;
+        ts.episode[i] = ep;
+        int slot = atomicAdd(ts.count, 1);
+        ts.list[slot] = i;
+        ts.seeds[slot] = (uint32_t)i + (uint32_t)tk.n * ep;
+        ts.diff[slot] = d.difficulty[i];
+        steps = 0;
+        atomicAdd(&ts.stats[cause], 1ull);
+    }
+    ts.torso_steps[i] = steps;
+}
+
+__global__ void k_fill_int(int *p, int v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+hipError_t launch_fill_int(int *p, int v, int n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts, hipStream_t s)
+{
+    if (d.n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_falls, dim3((d.n + 255) / 256), dim3(256), 0, s, d, m_dev, tk, ts);
     return hipGetLastError();
 }
 
@@ -284,6 +357,7 @@ __global__ void k_init_template(DevWorlds d, const TemplateK *__restrict__ tk, f
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
         d.outside[i] = outside;
+        d.difficulty[i] = 0.0f;
         d.err[i] = 0;
         for (int c = 0; c < d.cells; ++c)
             d.terrain[i * d.cells + c] = 0.0f;

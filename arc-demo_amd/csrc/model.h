@@ -5,6 +5,7 @@
  * host (arc_sim_create), then copied to device constant memory.
  */
 #pragma once
+#include <math.h>
 #include <vector>
 #include "xpbd_core.h"
 
@@ -63,6 +64,14 @@ V3 inertia_sphere(float m, float r);
 V3 inertia_capsule_z(float m, float r, float half);
 V3 inertia_box(float m, V3 he);
 void robot_build(HostWorld &w, RobotInfo &info);
+
+/* worst-case columns a sphere of radius r overlaps: per axis at most
+ * floor(2r / cell) + 2 indices (one more than the span, one for rounding) */
+inline int window_cells(float r, float cell)
+{
+    int k = (int)floor((double)(2.0f * r) / (double)cell) + 2;
+    return k * k;
+}
 
 /* Joint constants for a substep length h (derived products rounded in the
  * reference's evaluation order). */

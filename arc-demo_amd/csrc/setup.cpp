@@ -35,14 +35,6 @@ static bool topology_ok(const HostWorld &w)
     return true;
 }
 
-/* worst-case columns a sphere of radius r overlaps: per axis at most
- * floor(2r / cell) + 2 indices (one more than the span, one for rounding) */
-static int window_cells(float r, float cell)
-{
-    int k = (int)floor((double)(2.0f * r) / (double)cell) + 2;
-    return k * k;
-}
-
 bool sim_setup(SimSetup &S, double dt, int substeps, int grid, double cell)
 {
     if (substeps <= 0 || grid <= 0 || !(cell > 0.0) || !(dt > 0.0)) {

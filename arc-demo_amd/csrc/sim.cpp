@@ -16,6 +16,7 @@
 #include "device_state.h"
 #include "model.h"
 #include "setup.h"
+#include "wave_tables.h"
 #include "reset_engine.h"
 
 using namespace xpbd;
@@ -41,6 +42,13 @@ struct arc_sim {
     float *d_dump;
     arc_world *mirror;
     int block, grid_blocks;
+    int form;              /* 0: one world per lane, 1: one world per wavefront */
+    WaveK wave;
+    WaveK *d_wave;
+    TaskK task;
+    TaskState ts;
+    void *ts_mem;
+    int reset_blocks;
     int err;
     char msg[256];
 };
@@ -121,6 +129,14 @@ static bool ensure_pin(arc_sim *s, size_t bytes)
     return true;
 }
 
+/* the configured form of the step kernel over all worlds */
+static hipError_t launch_form(arc_sim *s, const double *td, const float *tf)
+{
+    if (s->form == 1)
+        return launch_step_wave(s->dev, s->d_model, s->d_wave, td, tf, 0, s->stream);
+    return launch_step(s->dev, s->d_model, td, tf, nullptr, 0, s->n, s->block, s->grid_blocks, s->stream);
+}
+
 extern "C" {
 
 int arcx_device_count(void)
@@ -157,6 +173,11 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         return nullptr;
     }
     const Model &M = s->su.model;
+    if (!build_wave_tables(s->su.tmpl, (float)cell, s->wave)) {
+        fprintf(stderr, "libarcsim: world exceeds the wavefront kernel's tables\n");
+        delete s;
+        return nullptr;
+    }
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess
         || (e = dev_alloc(s->dev, n_worlds, s->su.grid * s->su.grid, M.cslots)) != hipSuccess
@@ -164,6 +185,8 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         || (e = hipMalloc((void **)&s->d_reset, sizeof(ResetK))) != hipSuccess
         || (e = hipMalloc((void **)&s->d_tmpl, sizeof(TemplateK))) != hipSuccess
         || (e = hipMalloc((void **)&s->d_dump, 1024 * sizeof(float))) != hipSuccess
+        || (e = hipMalloc((void **)&s->d_wave, sizeof(WaveK))) != hipSuccess
+        || (e = hipMemcpyAsync(s->d_wave, &s->wave, sizeof(WaveK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
         || (e = hipMemcpyAsync(s->d_model, &M, sizeof(Model), hipMemcpyHostToDevice, s->stream)) != hipSuccess
         || (e = hipMemcpyAsync(s->d_reset, &s->su.rk, sizeof(ResetK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
         || (e = hipMemcpyAsync(s->d_tmpl, &s->su.tk, sizeof(TemplateK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
@@ -172,6 +195,34 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         fprintf(stderr, "libarcsim: arc_sim_create: %s\n", hipGetErrorString(e));
         arc_sim_destroy(s);
         return nullptr;
+    }
+    /* task defaults of arc/task.py:57-59 */
+    s->task.half_extent = 0.5 * s->su.grid * cell;
+    s->task.void_z = -2.0;
+    s->task.flip_up_z = 0.3;
+    s->task.beached_steps = 30;
+    s->task.n = n_worlds;
+    s->reset_blocks = 1024;
+    {
+        size_t N = (size_t)(n_worlds > 0 ? n_worlds : 1);
+        size_t bytes = N * (sizeof(int) * 3 + sizeof(uint32_t) * 2 + sizeof(float)) + 64 + 6 * sizeof(unsigned long long);
+        char *p = nullptr;
+        if ((e = hipMalloc((void **)&p, bytes)) != hipSuccess || (e = hipMemsetAsync(p, 0, bytes, s->stream)) != hipSuccess
+            || (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+            fprintf(stderr, "libarcsim: arc_sim_create: %s\n", hipGetErrorString(e));
+            arc_sim_destroy(s);
+            return nullptr;
+        }
+        s->ts_mem = p;
+        unsigned long long *st = (unsigned long long *)p;
+        s->ts.stats = st;
+        s->ts.count = (int *)(st + 6);
+        s->ts.torso_steps = (int *)(p + 64 + 6 * sizeof(unsigned long long));
+        s->ts.cause = s->ts.torso_steps + N;
+        s->ts.list = s->ts.cause + N;
+        s->ts.episode = (uint32_t *)(s->ts.list + N);
+        s->ts.seeds = s->ts.episode + N;
+        s->ts.diff = (float *)(s->ts.seeds + N);
     }
     return s;
 }
@@ -187,6 +238,8 @@ void arc_sim_destroy(arc_sim *s)
     if (s->d_reset) (void)hipFree(s->d_reset);
     if (s->d_tmpl) (void)hipFree(s->d_tmpl);
     if (s->d_dump) (void)hipFree(s->d_dump);
+    if (s->d_wave) (void)hipFree(s->d_wave);
+    if (s->ts_mem) (void)hipFree(s->ts_mem);
     if (s->d_io) (void)hipFree(s->d_io);
     if (s->d_aux) (void)hipFree(s->d_aux);
     if (s->h_pin) (void)hipHostFree(s->h_pin);
@@ -259,7 +312,8 @@ void arc_sim_reset(arc_sim *s, const int *ids, int n_ids, const unsigned int *se
     double *ddiff = (double *)(s->d_aux + (size_t)m * (sizeof(int) + sizeof(uint32_t)));
     /* the double block must be 8-byte aligned: m*8 bytes precede it */
     HIPCHK(s, hipMemcpyAsync(s->d_aux, s->h_pin, bytes, hipMemcpyHostToDevice, s->stream), "arc_sim_reset upload");
-    HIPCHK(s, launch_reset(s->dev, s->d_model, s->d_reset, did, dseed, ddiff, m, s->stream), "arc_sim_reset launch");
+    HIPCHK(s, launch_reset(s->dev, s->d_model, s->d_reset, did, dseed, ddiff, nullptr, m, nullptr, 4096, s->stream),
+           "arc_sim_reset launch");
     HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_reset");
 }
 
@@ -273,8 +327,7 @@ void arc_sim_step(arc_sim *s, const double *targets)
     HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
     memcpy(s->h_pin, targets, nt * sizeof(double));
     HIPCHK(s, hipMemcpyAsync(s->d_io, s->h_pin, nt * sizeof(double), hipMemcpyHostToDevice, s->stream), "arc_sim_step upload");
-    HIPCHK(s, launch_step(s->dev, s->d_model, s->d_io, nullptr, nullptr, 0, s->n, s->block, s->grid_blocks, s->stream),
-           "arc_sim_step launch");
+    HIPCHK(s, launch_form(s, s->d_io, nullptr), "arc_sim_step launch");
     HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
 }
 
@@ -427,8 +480,7 @@ int arcx_terrain(const arc_sim *cs, int i, double *out)
 
 int arcx_step_device(arc_sim *s, const float *targets_dev)
 {
-    HIPCHK_RET(s, launch_step(s->dev, s->d_model, nullptr, targets_dev, nullptr, 0, s->n, s->block, s->grid_blocks, s->stream),
-               "arcx_step_device", -1);
+    HIPCHK_RET(s, launch_form(s, nullptr, targets_dev), "arcx_step_device", -1);
     return 0;
 }
 
@@ -447,8 +499,7 @@ int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long s
     HIPCHK_RET(s, hipEventRecord(t0, s->stream), "arcx_time_steps", -1);
     for (int k = 0; k < steps; ++k) {
         HIPCHK_RET(s, hipEventRecord(e0, s->stream), "arcx_time_steps", -1);
-        HIPCHK_RET(s, launch_step(s->dev, s->d_model, nullptr, targets_dev + (size_t)k * step_floats, nullptr, 0, s->n,
-                                  s->block, s->grid_blocks, s->stream), "arcx_time_steps", -1);
+        HIPCHK_RET(s, launch_form(s, nullptr, targets_dev + (size_t)k * step_floats), "arcx_time_steps", -1);
         HIPCHK_RET(s, hipEventRecord(e1, s->stream), "arcx_time_steps", -1);
         HIPCHK_RET(s, hipEventSynchronize(e1), "arcx_time_steps", -1);
         float ms = 0.0f;
@@ -470,10 +521,75 @@ int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long s
     return 0;
 }
 
+int arcx_task_config(arc_sim *s, int beached_steps, double flip_up_z, double void_z)
+{
+    s->task.beached_steps = beached_steps;
+    s->task.flip_up_z = flip_up_z;
+    s->task.void_z = void_z;
+    return 0;
+}
+
+/* step, fall rules, auto-reset of the fallen worlds: no host round trip */
+static int step_task_impl(arc_sim *s, const float *targets_dev, const Range *ranges, int n_ranges, int total)
+{
+    HIPCHK_RET(s, launch_fill_int(s->ts.count, 0, 1, s->stream), "arcx_step_task", -1);
+    if (ranges || s->form == 0)
+        HIPCHK_RET(s, launch_step(s->dev, s->d_model, nullptr, targets_dev, ranges, n_ranges, total, s->block,
+                                  s->grid_blocks, s->stream), "arcx_step_task step", -1);
+    else
+        HIPCHK_RET(s, launch_form(s, nullptr, targets_dev), "arcx_step_task step", -1);
+    HIPCHK_RET(s, launch_falls(s->dev, s->d_model, s->task, s->ts, s->stream), "arcx_step_task falls", -1);
+    HIPCHK_RET(s, launch_reset(s->dev, s->d_model, s->d_reset, s->ts.list, s->ts.seeds, nullptr, s->ts.diff, 0,
+                               s->ts.count, s->reset_blocks, s->stream), "arcx_step_task reset", -1);
+    return 0;
+}
+
+int arcx_step_task(arc_sim *s, const float *targets_dev)
+{
+    return step_task_impl(s, targets_dev, nullptr, 0, s->n);
+}
+
+int arcx_task_causes(const arc_sim *cs, int *out)
+{
+    arc_sim *s = const_cast<arc_sim *>(cs);
+    HIPCHK_RET(s, hipMemcpyAsync(out, s->ts.cause, (size_t)s->n * sizeof(int), hipMemcpyDeviceToHost, s->stream),
+               "arcx_task_causes", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arcx_task_causes", -1);
+    return 0;
+}
+
+int arcx_task_stats(const arc_sim *cs, long long *out5)
+{
+    arc_sim *s = const_cast<arc_sim *>(cs);
+    unsigned long long v[5];
+    HIPCHK_RET(s, hipMemcpyAsync(v, s->ts.stats, sizeof(v), hipMemcpyDeviceToHost, s->stream), "arcx_task_stats", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arcx_task_stats", -1);
+    for (int k = 0; k < 5; ++k)
+        out5[k] = (long long)v[k];
+    return 0;
+}
+
+int arcx_task_episodes(const arc_sim *cs, unsigned int *out)
+{
+    arc_sim *s = const_cast<arc_sim *>(cs);
+    HIPCHK_RET(s, hipMemcpyAsync(out, s->ts.episode, (size_t)s->n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream),
+               "arcx_task_episodes", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arcx_task_episodes", -1);
+    return 0;
+}
+
 int arcx_sync(arc_sim *s)
 {
     HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arcx_sync", -1);
     return s->err;
+}
+
+int arcx_set_form(arc_sim *s, int form)
+{
+    if (form < 0 || form > 1)
+        return -1;
+    s->form = form;
+    return 0;
 }
 
 int arcx_set_launch(arc_sim *s, int block, int grid_blocks)

@@ -1,0 +1,274 @@
+/*
+ * wave_kernel.hip -- the plain form of the step: one world per wavefront,
+ * one launch per control step.
+ *
+ * The world's bodies, joint accumulators, sphere anchors and the substep's
+ * contact list are staged in LDS.  Lanes work in parallel where the reference
+ * loop iterations are independent:
+ *   - integration and velocity update: one lane per body;
+ *   - joint projection (csrc/arc_phys.c:729-730, 734-735): Gauss-Seidel in the
+ *     reference order, executed level by level of its dependency DAG -- a
+ *     joint only reads and writes its parent and child, so joints with no body
+ *     in common commute and each level runs on parallel lanes, giving the
+ *     serial order's results bit for bit;
+ *   - sphere contacts: one lane per body, each walking that body's spheres in
+ *     order (a contact solve moves only the sphere's own body,
+ *     csrc/arc_phys.c:488-500);
+ *   - contact velocities: one lane per body, its contacts in record order.
+ * The topology comes from tables built on the host from the world itself, and
+ * every body is the run-time-kinded KGEN, so the code follows the reference's
+ * generic arithmetic literally.
+ */
+#include "device_state.h"
+#include "wave_tables.h"
+
+namespace xpbd {
+
+struct WaveWorld {
+    Body b[WAVE_MAX_BODIES];
+    float target[WAVE_MAX_JOINTS], tacc[WAVE_MAX_JOINTS];
+    float angle[WAVE_MAX_JOINTS], rate[WAVE_MAX_JOINTS], torque[WAVE_MAX_JOINTS];
+    int touch[WAVE_MAX_SPHERES], anchored[WAVE_MAX_SPHERES];
+    V3 anc[WAVE_MAX_SPHERES], ancn[WAVE_MAX_SPHERES];
+    float cl[WAVE_MAX_SLOTS][6];
+    int ccount[WAVE_MAX_BODIES];
+    int unstable;
+};
+
+template <int PK>
+__device__ __forceinline__ void wave_joint_pos(WaveWorld &W, const Model &M, const WaveK &T, int j)
+{
+    int p = T.jp[j], c = T.jc[j];
+    Body C = W.b[c];
+    if (PK >= 0) {
+        Body P = W.b[p];
+        joint_positions<KGEN, KGEN>(&P, &M.body[p], C, M.body[c], M.joint[j], W.target[j], M.hh, W.tacc[j]);
+        W.b[p] = P;
+    } else {
+        joint_positions<-1, KGEN>(nullptr, nullptr, C, M.body[c], M.joint[j], W.target[j], M.hh, W.tacc[j]);
+    }
+    W.b[c] = C;
+}
+
+template <int PK>
+__device__ __forceinline__ void wave_joint_vel(WaveWorld &W, const Model &M, const WaveK &T, int j)
+{
+    int p = T.jp[j], c = T.jc[j];
+    Body C = W.b[c];
+    if (PK >= 0) {
+        Body P = W.b[p];
+        joint_velocities<KGEN, KGEN>(&P, &M.body[p], C, M.body[c], M.joint[j], M.h, W.tacc[j]);
+        W.b[p] = P;
+    } else {
+        joint_velocities<-1, KGEN>(nullptr, nullptr, C, M.body[c], M.joint[j], M.h, W.tacc[j]);
+    }
+    W.b[c] = C;
+}
+
+/* spheres of contact-body group g, in sphere order, against the terrain */
+__device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M, const WaveK &T, int g,
+                                                   const TerrainRef &tr, int &total)
+{
+    const int B = T.cb_body[g];
+    Body b = W.b[B];
+    int &cnt = W.ccount[B];
+    for (int k = T.cb_first[g]; k < T.cb_first[g + 1]; ++k) {
+        const int s = T.cb_sphere[k];
+        auto rec = [&](V3 l, V3 n) {
+            if (total < MAXC) {
+                if (cnt < T.cb_cap[g]) {
+                    float *c = W.cl[T.cb_slot[g] + cnt];
+                    c[0] = l.x; c[1] = l.y; c[2] = l.z; c[3] = n.x; c[4] = n.y; c[5] = n.z;
+                }
+                cnt++;
+                total++;
+            }
+        };
+        collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], W.anc[s],
+                              W.ancn[s], M.terr, tr, rec, M.mus, M.mud);
+    }
+    W.b[B] = b;
+}
+
+template <class TT>
+__global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__restrict__ m, const WaveK *__restrict__ wk,
+                                                  const TT *__restrict__ targets, int total_worlds)
+{
+    __shared__ WaveWorld W;
+    const Model &M = *m;
+    const WaveK &T = *wk;
+    const int lane = threadIdx.x;
+    const size_t S = d.stride;
+    for (int i = blockIdx.x; i < total_worlds; i += gridDim.x) {
+        /* targets clamped to the limits (every world), csrc/arc_sim.c:132-134 */
+        if (lane < T.nj) {
+            float tg = clampf_((float)targets[(size_t)i * T.nj + lane], M.joint[lane].lo, M.joint[lane].hi);
+            W.target[lane] = tg;
+            d.target[lane * S + i] = tg;
+        }
+        if (lane == 0)
+            W.unstable = d.unstable[i];
+        __syncthreads();
+        if (W.unstable) {
+            __syncthreads();
+            continue;
+        }
+        /* stage the world in LDS; prologue of arc_world_step */
+        if (lane < T.nb) {
+            Body &b = W.b[lane];
+            b.x = v3(d.x[(lane * 3 + 0) * S + i], d.x[(lane * 3 + 1) * S + i], d.x[(lane * 3 + 2) * S + i]);
+            b.q = q4(d.q[(lane * 4 + 0) * S + i], d.q[(lane * 4 + 1) * S + i], d.q[(lane * 4 + 2) * S + i],
+                     d.q[(lane * 4 + 3) * S + i]);
+            b.v = v3(d.v[(lane * 3 + 0) * S + i], d.v[(lane * 3 + 1) * S + i], d.v[(lane * 3 + 2) * S + i]);
+            b.w = v3(d.w[(lane * 3 + 0) * S + i], d.w[(lane * 3 + 1) * S + i], d.w[(lane * 3 + 2) * S + i]);
+            Body t = b;
+            bsync<KGEN>(t, M.body[lane]);
+            b = t;
+            W.ccount[lane] = 0;
+        }
+        if (lane < T.ns) {
+            uint32_t an = d.anchored[i];
+            W.touch[lane] = 0;
+            W.anchored[lane] = (an >> lane) & 1u;
+            W.anc[lane] = v3(d.anc[(lane * 3 + 0) * S + i], d.anc[(lane * 3 + 1) * S + i], d.anc[(lane * 3 + 2) * S + i]);
+            W.ancn[lane] = v3(d.ancn[(lane * 3 + 0) * S + i], d.ancn[(lane * 3 + 1) * S + i], d.ancn[(lane * 3 + 2) * S + i]);
+        }
+        if (lane < T.nj)
+            W.tacc[lane] = 0.0f;
+        __syncthreads();
+
+        TerrainRef tr;
+        tr.h = d.terrain + (size_t)i * d.cells;
+        tr.outside = d.outside[i];
+
+        for (int sub = 0; sub < M.substeps; ++sub) {
+            if (lane < T.nb) {
+                Body b = W.b[lane];
+                integrate<KGEN>(b, M.body[lane], M.g, M.h);
+                W.b[lane] = b;
+                W.ccount[lane] = 0;
+            }
+            __syncthreads();
+            for (int L = 0; L < T.nlev; ++L) {
+                int k = T.lev_first[L] + lane;
+                if (k < T.lev_first[L + 1]) {
+                    int j = T.lev_joint[k];
+                    if (T.jp[j] >= 0)
+                        wave_joint_pos<KGEN>(W, M, T, j);
+                    else
+                        wave_joint_pos<-1>(W, M, T, j);
+                }
+                __syncthreads();
+            }
+            if (T.serial_collide) {
+                /* the contact-record cap could bind: keep the global sphere order */
+                if (lane == 0) {
+                    int total = 0;
+                    for (int g = 0; g < T.ncb; ++g)
+                        wave_collide_group(W, M, T, g, tr, total);
+                }
+            } else if (lane < T.ncb) {
+                int total = 0; /* the cap cannot bind: groups are independent */
+                wave_collide_group(W, M, T, lane, tr, total);
+            }
+            __syncthreads();
+            if (lane < T.nb) {
+                Body b = W.b[lane];
+                velocities(b, M.inv_h, M.two_h);
+                W.b[lane] = b;
+            }
+            __syncthreads();
+            for (int L = 0; L < T.nlev; ++L) {
+                int k = T.lev_first[L] + lane;
+                if (k < T.lev_first[L + 1]) {
+                    int j = T.lev_joint[k];
+                    if (T.jp[j] >= 0)
+                        wave_joint_vel<KGEN>(W, M, T, j);
+                    else
+                        wave_joint_vel<-1>(W, M, T, j);
+                }
+                __syncthreads();
+            }
+            if (lane < T.ncb) {
+                const int B = T.cb_body[lane];
+                Body b = W.b[B];
+                int n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
+                for (int k = 0; k < n; ++k) {
+                    const float *c = W.cl[T.cb_slot[lane] + k];
+                    contact_velocity<KGEN>(b, M.body[B], v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]));
+                }
+                W.b[B] = b;
+            }
+            __syncthreads();
+        }
+        /* epilogue: outputs, time, stability */
+        if (lane < T.nj) {
+            int p = T.jp[lane], c = T.jc[lane];
+            Body C = W.b[c];
+            if (p >= 0) {
+                Body P = W.b[p];
+                W.angle[lane] = joint_angle<true>(&P, C, M.joint[lane]);
+                W.rate[lane] = joint_rate<true>(&P, C, M.joint[lane]);
+            } else {
+                W.angle[lane] = joint_angle<false>(nullptr, C, M.joint[lane]);
+                W.rate[lane] = joint_rate<false>(nullptr, C, M.joint[lane]);
+            }
+            W.torque[lane] = W.tacc[lane] / (float)M.substeps;
+        }
+        bool bad = lane < T.nb && body_unstable(W.b[lane]);
+        bool any_bad = __any(bad);
+        __syncthreads();
+        /* write back */
+        if (lane < T.nb) {
+            const Body &b = W.b[lane];
+            d.x[(lane * 3 + 0) * S + i] = b.x.x; d.x[(lane * 3 + 1) * S + i] = b.x.y; d.x[(lane * 3 + 2) * S + i] = b.x.z;
+            d.q[(lane * 4 + 0) * S + i] = b.q.w; d.q[(lane * 4 + 1) * S + i] = b.q.x;
+            d.q[(lane * 4 + 2) * S + i] = b.q.y; d.q[(lane * 4 + 3) * S + i] = b.q.z;
+            d.v[(lane * 3 + 0) * S + i] = b.v.x; d.v[(lane * 3 + 1) * S + i] = b.v.y; d.v[(lane * 3 + 2) * S + i] = b.v.z;
+            d.w[(lane * 3 + 0) * S + i] = b.w.x; d.w[(lane * 3 + 1) * S + i] = b.w.y; d.w[(lane * 3 + 2) * S + i] = b.w.z;
+        }
+        if (lane < T.nj) {
+            d.jout[(0 * NJ + lane) * S + i] = W.angle[lane];
+            d.jout[(1 * NJ + lane) * S + i] = W.rate[lane];
+            d.jout[(2 * NJ + lane) * S + i] = W.torque[lane];
+        }
+        if (lane < T.ns) {
+            d.touch[lane * S + i] = (uint8_t)W.touch[lane];
+            d.anc[(lane * 3 + 0) * S + i] = W.anc[lane].x; d.anc[(lane * 3 + 1) * S + i] = W.anc[lane].y;
+            d.anc[(lane * 3 + 2) * S + i] = W.anc[lane].z;
+            d.ancn[(lane * 3 + 0) * S + i] = W.ancn[lane].x; d.ancn[(lane * 3 + 1) * S + i] = W.ancn[lane].y;
+            d.ancn[(lane * 3 + 2) * S + i] = W.ancn[lane].z;
+        }
+        unsigned long long an = __ballot(lane < T.ns && W.anchored[lane]);
+        int ov = 0;
+        if (lane < T.ncb)
+            ov = W.ccount[T.cb_body[lane]] > T.cb_cap[lane] ? (1 << lane) : 0;
+        for (int off = 32; off > 0; off >>= 1)
+            ov |= __shfl_xor(ov, off);
+        if (lane == 0) {
+            d.anchored[i] = (uint32_t)an;
+            d.time[i] = d.time[i] + M.dt;
+            if (any_bad)
+                d.unstable[i] = 1;
+            if (ov)
+                d.err[i] |= ov;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_step_wave(const DevWorlds &d, const Model *m_dev, const WaveK *wk_dev, const double *targets_d,
+                            const float *targets_f, int grid_blocks, hipStream_t s)
+{
+    if (d.n <= 0)
+        return hipSuccess;
+    if (grid_blocks <= 0)
+        grid_blocks = d.n;
+    if (targets_d)
+        hipLaunchKernelGGL(k_step_wave<double>, dim3(grid_blocks), dim3(64), 0, s, d, m_dev, wk_dev, targets_d, d.n);
+    else
+        hipLaunchKernelGGL(k_step_wave<float>, dim3(grid_blocks), dim3(64), 0, s, d, m_dev, wk_dev, targets_f, d.n);
+    return hipGetLastError();
+}
+
+} /* namespace xpbd */

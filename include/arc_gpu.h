@@ -48,6 +48,26 @@ int arcx_sync(arc_sim *s);
 int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long step_floats, double *kernel_ms,
                     double *wall_ms);
 
+/* Task layer on the device (arc/task.py:180-203, arc/env.py:99-117).
+ * arcx_step_task = one control step, then the fall rules in the task's
+ * precedence (unstable or non-finite > void > flip > beached), then an
+ * auto-reset of every fallen world with seed i + N * episode_i (episode_i
+ * counts world i's auto-resets) and the difficulty of its last reset.
+ * Asynchronous; the task's torso-contact counters persist across calls. */
+int arcx_task_config(arc_sim *s, int beached_steps, double flip_up_z, double void_z);
+int arcx_step_task(arc_sim *s, const float *targets_dev);
+/* causes of the last arcx_step_task per world: 0 none, 1 unstable or
+ * non-finite, 2 void, 3 flip, 4 beached (synchronous copy) */
+int arcx_task_causes(const arc_sim *s, int *out);
+/* terminations so far per cause (index 0 unused) */
+int arcx_task_stats(const arc_sim *s, long long *out5);
+int arcx_task_episodes(const arc_sim *s, unsigned int *out);
+
+/* Form of the step kernel: 0 = one world per lane (default; all substeps
+ * of a control step in one launch, world state in registers), 1 = the plain
+ * form, one world per wavefront with the world staged in LDS. */
+int arcx_set_form(arc_sim *s, int form);
+
 /* Launch shape of the step kernel: threads per block (a multiple of 64) and
  * blocks (0 = one pass over the worlds). */
 int arcx_set_launch(arc_sim *s, int block, int grid_blocks);

@@ -150,3 +150,26 @@ def test_world_mirror(native):
     assert raw[0] == st[0] and raw[1] == st[1] and raw[2] == st[2]
     counts = (ctypes.c_int * 6).from_address(p + 40)
     assert list(counts)[:3] == [13, 12, 21]
+
+
+@pytest.mark.parametrize("difficulty", [0.0, "mixed"])
+def test_wave_form_matches_lane_form(native, difficulty):
+    """The plain form (one world per wavefront, joints by dependency level,
+    generic arithmetic) computes the lane form's results: equal values,
+    signed zeros aside; and both stay within the bar of the oracle."""
+    n = 96
+    diff = P.difficulties(1, n) if difficulty == "mixed" else np.full(n, difficulty)
+    lane, o = _pair(native, n, diff)
+    wave = native.Sim(n)
+    wave.set_form("wave")
+    ids = np.arange(n, dtype=np.int32)
+    wave.reset(ids, ids.astype(np.uint32), diff)
+    ts = action_stream(n, 3)
+    for k in range(3):
+        lane.step(ts[k])
+        wave.step(ts[k])
+        o.step(ts[k])
+        a, b = _states(wave, n), _states(lane, n)
+        assert same(a, b).all(), f"wave vs lane after step {k + 1}"
+    c = _states(o, n)
+    assert np.all(np.abs(a - c) / (np.abs(c) + 1.0) <= 1e-4)
