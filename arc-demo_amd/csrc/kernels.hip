@@ -78,10 +78,6 @@ __device__ __forceinline__ void load_world(const DevWorlds &d, size_t i, LaneWor
         W.b[b].w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
     });
     W.anchored = d.anchored[i];
-    static_for<0, NS>([&](auto s) {
-        W.anc[s] = v3(d.anc[(s * 3 + 0) * S + i], d.anc[(s * 3 + 1) * S + i], d.anc[(s * 3 + 2) * S + i]);
-        W.ancn[s] = v3(d.ancn[(s * 3 + 0) * S + i], d.ancn[(s * 3 + 1) * S + i], d.ancn[(s * 3 + 2) * S + i]);
-    });
     W.unstable = d.unstable[i];
     W.time = d.time[i];
 }
@@ -102,20 +98,29 @@ __device__ __forceinline__ void store_world(const DevWorlds &d, size_t i, const 
         d.jout[(2 * NJ + j) * S + i] = W.torque[j];
     });
     d.anchored[i] = W.anchored;
-    static_for<0, NS>([&](auto s) {
-        d.touch[s * S + i] = (uint8_t)W.touch[s];
-        d.anc[(s * 3 + 0) * S + i] = W.anc[s].x; d.anc[(s * 3 + 1) * S + i] = W.anc[s].y; d.anc[(s * 3 + 2) * S + i] = W.anc[s].z;
-        d.ancn[(s * 3 + 0) * S + i] = W.ancn[s].x; d.ancn[(s * 3 + 1) * S + i] = W.ancn[s].y; d.ancn[(s * 3 + 2) * S + i] = W.ancn[s].z;
-    });
+    static_for<0, NS>([&](auto s) { d.touch[s * S + i] = (uint8_t)touch_of(W, s); });
     d.unstable[i] = W.unstable;
     d.time[i] = W.time;
 }
 
-template <class TT>
-__global__ __launch_bounds__(256) void k_step(DevWorlds d, const Model *__restrict__ m, const TT *__restrict__ targets,
-                                              const Range *__restrict__ ranges, int n_ranges, int total)
+/* friction anchors of the block's worlds in LDS: [sphere][thread] */
+template <int BLOCK> struct LdsAnchors {
+    V3 *a, *n;
+    int tid;
+    __device__ __forceinline__ V3 &anc(int s) { return a[s * BLOCK + tid]; }
+    __device__ __forceinline__ V3 &ancn(int s) { return n[s * BLOCK + tid]; }
+};
+
+template <class TT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__restrict__ m, const TT *__restrict__ targets,
+                                                const Range *__restrict__ ranges, int n_ranges, int total)
 {
+    __shared__ V3 s_anc[NS * BLOCK], s_ancn[NS * BLOCK];
     const Model &M = *m;
+    LdsAnchors<BLOCK> A;
+    A.a = s_anc;
+    A.n = s_ancn;
+    A.tid = threadIdx.x;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         int i = t;
         if (ranges) {
@@ -135,6 +140,10 @@ __global__ __launch_bounds__(256) void k_step(DevWorlds d, const Model *__restri
         if (d.unstable[i])
             continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
         load_world(d, i, W);
+        static_for<0, NS>([&](auto s) {
+            A.anc(s) = v3(d.anc[(s * 3 + 0) * S + i], d.anc[(s * 3 + 1) * S + i], d.anc[(s * 3 + 2) * S + i]);
+            A.ancn(s) = v3(d.ancn[(s * 3 + 0) * S + i], d.ancn[(s * 3 + 1) * S + i], d.ancn[(s * 3 + 2) * S + i]);
+        });
         TerrainRef tr;
         tr.h = d.terrain + (size_t)i * d.cells;
         tr.outside = d.outside[i];
@@ -142,8 +151,13 @@ __global__ __launch_bounds__(256) void k_step(DevWorlds d, const Model *__restri
         cs.base = d.cbuf;
         cs.stride = S;
         cs.world = i;
-        int ov = lane_step(W, M, tr, cs);
+        int ov = lane_step(W, A, M, tr, cs);
         store_world(d, i, W);
+        static_for<0, NS>([&](auto s) {
+            V3 a = A.anc(s), n = A.ancn(s);
+            d.anc[(s * 3 + 0) * S + i] = a.x; d.anc[(s * 3 + 1) * S + i] = a.y; d.anc[(s * 3 + 2) * S + i] = a.z;
+            d.ancn[(s * 3 + 0) * S + i] = n.x; d.ancn[(s * 3 + 1) * S + i] = n.y; d.ancn[(s * 3 + 2) * S + i] = n.z;
+        });
         if (ov)
             d.err[i] |= ov;
     }
@@ -158,10 +172,23 @@ hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *tar
         block = 64;
     if (grid_blocks <= 0)
         grid_blocks = (total + block - 1) / block;
-    if (targets_d)
-        hipLaunchKernelGGL(k_step<double>, dim3(grid_blocks), dim3(block), 0, s, d, m_dev, targets_d, ranges, n_ranges, total);
-    else
-        hipLaunchKernelGGL(k_step<float>, dim3(grid_blocks), dim3(block), 0, s, d, m_dev, targets_f, ranges, n_ranges, total);
+#define ARC_STEP_LAUNCH(B)                                                                                     \
+    case B:                                                                                                    \
+        if (targets_d)                                                                                         \
+            hipLaunchKernelGGL((k_step<double, B>), dim3(grid_blocks), dim3(B), 0, s, d, m_dev, targets_d, ranges,   \
+                               n_ranges, total);                                                               \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_step<float, B>), dim3(grid_blocks), dim3(B), 0, s, d, m_dev, targets_f, ranges,    \
+                               n_ranges, total);                                                               \
+        break;
+    switch (block) {
+        ARC_STEP_LAUNCH(64)
+        ARC_STEP_LAUNCH(128)
+        ARC_STEP_LAUNCH(256)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef ARC_STEP_LAUNCH
     return hipGetLastError();
 }
 

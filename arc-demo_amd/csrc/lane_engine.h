@@ -58,16 +58,26 @@ struct Model {
     int cslots;
 };
 
-/* Register-resident state of one world during a step. */
+/* Register-resident state of one world during a step.  The spheres'
+ * friction anchors live in an AnchorStore (LDS on the device), and the
+ * per-sphere contact counts of the step are packed four to a register. */
+constexpr int NTP = (NS + 3) / 4;
 struct LaneWorld {
     Body b[NB];
     float target[NJ], tacc[NJ];
     float angle[NJ], rate[NJ], torque[NJ];
-    int touch[NS];
+    uint32_t touchp[NTP]; /* byte s % 4 of word s / 4: substeps sphere s touched */
     uint32_t anchored;    /* bit s: sphere s has a valid friction anchor */
-    V3 anc[NS], ancn[NS];
     int unstable;
     float time;
+};
+XD int touch_of(const LaneWorld &W, int s) { return (int)((W.touchp[s >> 2] >> (8 * (s & 3))) & 0xffu); }
+
+/* anchors of the host build: plain arrays */
+struct ArrayAnchors {
+    V3 a[NS], n[NS];
+    XD V3 &anc(int s) { return a[s]; }
+    XD V3 &ancn(int s) { return n[s]; }
 };
 
 /* Contact list storage: slot k field f of this world. */
@@ -92,12 +102,13 @@ struct ContactStore {
 /* One control step (arc_world_step, csrc/arc_phys.c:707-747).
  * Returns a bit mask of bodies whose contact list overflowed its slots
  * (never expected for the robot; reported as an error by the host). */
-XD int lane_step(LaneWorld &W, const Model &M, const TerrainRef &tr, const ContactStore &cs)
+template <class Anchors>
+XD int lane_step(LaneWorld &W, Anchors &A, const Model &M, const TerrainRef &tr, const ContactStore &cs)
 {
     int overflow = 0;
     /* prologue: refresh caches, clear counters (lim is precomputed per joint) */
     static_for<0, NB>([&](auto bi) { bsync<body_kind(bi)>(W.b[bi], M.body[bi]); });
-    static_for<0, NS>([&](auto si) { W.touch[si] = 0; });
+    static_for<0, NTP>([&](auto k) { W.touchp[k] = 0u; });
     static_for<0, NJ>([&](auto ji) { W.tacc[ji] = 0.0f; });
 
     for (int sub = 0; sub < M.substeps; ++sub) {
@@ -120,6 +131,7 @@ XD int lane_step(LaneWorld &W, const Model &M, const TerrainRef &tr, const Conta
             constexpr int B = sphere_body(si);
             constexpr int CI = cbody_index(B);
             int anch = (W.anchored >> si) & 1u;
+            int hit = 0;
             auto rec = [&](V3 l, V3 n) {
                 if (total < MAXC) {
                     if (ncount[CI] < M.ccap[CI])
@@ -131,9 +143,10 @@ XD int lane_step(LaneWorld &W, const Model &M, const TerrainRef &tr, const Conta
                 }
             };
             collide<body_kind(B), sphere_at(si)>(W.b[B], M.body[B], M.sphere[si].at, M.sphere[si].rad,
-                                                 W.touch[si], anch, W.anc[si], W.ancn[si], M.terr, tr, rec,
+                                                 hit, anch, A.anc(si), A.ancn(si), M.terr, tr, rec,
                                                  M.mus, M.mud);
             W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
+            W.touchp[si >> 2] += (uint32_t)hit << (8 * (si & 3));
         });
 
         static_for<0, NB>([&](auto bi) { velocities(W.b[bi], M.inv_h, M.two_h); });

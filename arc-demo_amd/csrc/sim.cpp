@@ -424,6 +424,22 @@ void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out)
         out[i] = r.next();
 }
 
+void *arcx_device_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+        return nullptr;
+    return p;
+}
+
+int arcx_device_free(void *p) { return p ? (int)hipFree(p) : 0; }
+
+int arcx_memcpy(void *dst, const void *src, size_t bytes, int to_device)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+    return (int)e;
+}
+
 int arcx_state_size(void) { return NB * 13 + NJ * 4 + NS * 8 + 2; }
 
 int arcx_state(const arc_sim *cs, int i, double *out)
@@ -594,7 +610,7 @@ int arcx_set_form(arc_sim *s, int form)
 
 int arcx_set_launch(arc_sim *s, int block, int grid_blocks)
 {
-    if (block <= 0 || block % 64 || block > 256 || grid_blocks < 0)
+    if ((block != 64 && block != 128 && block != 256) || grid_blocks < 0)
         return -1;
     s->block = block;
     s->grid_blocks = grid_blocks;

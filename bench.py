@@ -54,7 +54,6 @@ def main() -> int:
               "worlds shard by HBM budget across GPUs with no exchange inside a step", file=sys.stderr)
         return 2
 
-    import torch
     from arc import native
 
     n = args.worlds
@@ -69,9 +68,8 @@ def main() -> int:
     u = _rng_stream(0, n * NJ * (steps + warm))
     a = (u >> 8).astype(np.float64) * (1.0 / 8388608.0) - 1.0
     targets = (np.tile(sim.default_angles, n * (steps + warm))[: a.size] + 0.6 * a).astype(np.float32)
-    dev = torch.from_numpy(targets).to("cuda")
-    torch.cuda.synchronize()
-    base = dev.data_ptr()
+    dev = native.DeviceBuffer.from_array(targets)
+    base = dev.ptr
     per = n * NJ
     # warmup
     if warm:

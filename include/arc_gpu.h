@@ -9,6 +9,7 @@
 #ifndef ARC_GPU_H
 #define ARC_GPU_H
 
+#include <stddef.h>
 #include "arc_sim.h"
 
 #ifdef __cplusplus
@@ -27,6 +28,13 @@ int arcx_device_count(void);
 /* n successive outputs of the engine's xorshift128 generator seeded `seed`
  * (the stream the reset draws from; also used to make synthetic inputs). */
 void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out);
+
+/* Plain device memory for inputs kept on the GPU (synthetic targets,
+ * client buffers): hipMalloc / hipFree / synchronous copies.  to_device = 1
+ * copies host -> device, 0 device -> host.  Returns NULL / a hipError_t. */
+void *arcx_device_alloc(size_t bytes);
+int arcx_device_free(void *p);
+int arcx_memcpy(void *dst, const void *src, size_t bytes, int to_device);
 
 /* Size of the per-world flat state of arcx_state (same layout as the
  * oracle's dump): 13 bodies x (x3 q4 v3 w3), 12 joints x (target angle rate

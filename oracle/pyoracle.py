@@ -100,7 +100,7 @@ class Lib:
             "orc_sim_step": (None, _p, _p, _i),
             "orc_sim_observe": (None, _p, _p, _p, _p, _p, _p),
             "orc_sim_heights": (None, _p, _p, _i, _p),
-            "orc_sim_terrain_at": (None, _p, _p, _i, _p, _p),
+            "orc_sim_terrain_at": (None, _p, _p, _p, _i, _p),
             "orc_state_size": (_i,),
             "orc_sim_state": (None, _p, _i, _p),
             "orc_sim_terrain": (None, _p, _i, _p),

@@ -18,6 +18,7 @@ struct HC {
     int n;
     SimSetup su;
     std::vector<LaneWorld> w;
+    std::vector<ArrayAnchors> anc;
     std::vector<float> terrain, outside;
     std::vector<float> cbuf;
     std::vector<int> err;
@@ -35,6 +36,8 @@ HC *hc_create(int n, double dt, int substeps, int grid, double cell)
     }
     const int cells = h->su.grid * h->su.grid;
     h->w.resize(n);
+    h->anc.assign(n, ArrayAnchors());
+    memset(h->anc.data(), 0, sizeof(ArrayAnchors) * (size_t)n);
     h->terrain.assign((size_t)n * cells, 0.0f);
     h->outside.assign(n, h->su.tmpl.outside);
     h->cbuf.assign((size_t)h->su.model.cslots * 6, 0.0f);
@@ -89,6 +92,7 @@ void hc_reset(HC *h, const int *ids, int n_ids, const unsigned *seeds, const dou
         pose_bodies(M, v3(0.0f, 0.0f, ground + R.stand + 0.03f), qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]), ang, bx, bq);
         LaneWorld &W = h->w[i];
         memset(&W, 0, sizeof(W));
+        memset(&h->anc[i], 0, sizeof(ArrayAnchors));
         for (int b = 0; b < NB; ++b) {
             W.b[b].x = bx[b];
             W.b[b].q = bq[b];
@@ -118,7 +122,7 @@ void hc_step(HC *h, const double *targets)
         cs.base = h->cbuf.data();
         cs.stride = 1;
         cs.world = 0;
-        h->err[i] |= lane_step(W, M, tr, cs);
+        h->err[i] |= lane_step(W, h->anc[i], M, tr, cs);
     }
 }
 
@@ -135,11 +139,12 @@ void hc_state(const HC *h, int i, double *out)
     for (int j = 0; j < NJ; ++j) {
         out[o++] = W.target[j]; out[o++] = W.angle[j]; out[o++] = W.rate[j]; out[o++] = W.torque[j];
     }
+    const ArrayAnchors &A = h->anc[i];
     for (int s = 0; s < NS; ++s) {
-        out[o++] = W.touch[s];
+        out[o++] = touch_of(W, s);
         out[o++] = (W.anchored >> s) & 1u;
-        out[o++] = W.anc[s].x; out[o++] = W.anc[s].y; out[o++] = W.anc[s].z;
-        out[o++] = W.ancn[s].x; out[o++] = W.ancn[s].y; out[o++] = W.ancn[s].z;
+        out[o++] = A.a[s].x; out[o++] = A.a[s].y; out[o++] = A.a[s].z;
+        out[o++] = A.n[s].x; out[o++] = A.n[s].y; out[o++] = A.n[s].z;
     }
     out[o++] = W.unstable;
     out[o++] = W.time;
@@ -155,4 +160,14 @@ void hc_terrain(const HC *h, int i, double *out)
 
 int hc_error(const HC *h, int i) { return h->err[i]; }
 
+}
+
+extern "C" void hc_joint_consts(const HC *h, int j, double *o)
+{
+    const JointK &k = h->su.model.joint[j];
+    const V3 v[6] = { k.r1, k.r2, k.a1, k.a2, k.b1, k.b2 };
+    for (int i = 0; i < 6; ++i) {
+        o[3 * i] = v[i].x; o[3 * i + 1] = v[i].y; o[3 * i + 2] = v[i].z;
+    }
+    o[18] = k.qrel0.w; o[19] = k.qrel0.x; o[20] = k.qrel0.y; o[21] = k.qrel0.z;
 }

@@ -17,7 +17,6 @@ pytestmark = pytest.mark.gpu
 
 
 def test_falls_and_autoreset_match_oracle():
-    import torch
     from arc import native
 
     n, steps = 96, 6
@@ -30,12 +29,13 @@ def test_falls_and_autoreset_match_oracle():
     o.reset(ids, ids.astype(np.uint32), diff)
     g.task_config(beached, flip, void)
     ts = action_stream(n, steps, seed=2)
-    dev = torch.from_numpy(ts.astype(np.float32).reshape(steps, -1)).cuda()
+    dev = native.DeviceBuffer.from_array(ts.astype(np.float32).reshape(steps, -1))
+    per = n * 12 * 4
     torso = np.zeros(n, np.int32)
     episode = np.zeros(n, np.int64)
     total = 0
     for k in range(steps):
-        g.step_task(dev[k].data_ptr())
+        g.step_task(dev.ptr + k * per)
         g.sync()
         o.step(ts[k])
         cause_o = o.falls(torso, beached, flip, void)

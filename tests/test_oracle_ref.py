@@ -160,3 +160,21 @@ def test_live_reference_partial_resets(prec):
             for s in (o, r):
                 s.reset(sub, np.array([99, 100, 101], np.uint32), np.array([0.0, 0.3, 1.0]))
         assert _eq(o.states(), r.states())
+
+
+@pytest.mark.parametrize("prec", ["32", "64s"])
+def test_live_reference_terrain_at(prec):
+    """arc_sim_terrain_at over points inside, on edges of and outside the grid."""
+    _need("r" + prec)
+    okind = "o" + prec.rstrip("s")
+    n = 6
+    o, r = P.Lib(okind).sim(n), P.Lib("r" + prec).sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    for s in (o, r):
+        s.reset(ids, ids + 3, np.linspace(0, 1, n))
+    m = 200
+    rng = np.random.default_rng(0)
+    pid = rng.integers(0, n, m).astype(np.int32)
+    xy = rng.uniform(-14, 14, (m, 2)).astype(np.float32).astype(np.float64)
+    xy[:10] = np.round(xy[:10] / 0.4) * 0.4  # column edges
+    assert _eq(o.terrain_at(pid, xy), r.terrain_at(pid, xy))
