@@ -103,24 +103,73 @@ __device__ __forceinline__ void store_world(const DevWorlds &d, size_t i, const 
     d.time[i] = W.time;
 }
 
-/* friction anchors of the block's worlds in LDS: [sphere][thread] */
-template <int BLOCK> struct LdsAnchors {
-    V3 *a, *n;
-    int tid;
-    __device__ __forceinline__ V3 &anc(int s) { return a[s * BLOCK + tid]; }
-    __device__ __forceinline__ V3 &ancn(int s) { return n[s * BLOCK + tid]; }
+/* friction anchor of sphere s of world i, read and written in its HBM
+ * arrays only while the sphere is in contact */
+struct GlobalAnchor {
+    float *a, *n;
+    size_t S, i;
+    int s;
+    __device__ __forceinline__ V3 get(const float *p) const
+    {
+        return v3(p[(s * 3 + 0) * S + i], p[(s * 3 + 1) * S + i], p[(s * 3 + 2) * S + i]);
+    }
+    __device__ __forceinline__ void put(float *p, V3 v)
+    {
+        p[(s * 3 + 0) * S + i] = v.x;
+        p[(s * 3 + 1) * S + i] = v.y;
+        p[(s * 3 + 2) * S + i] = v.z;
+    }
+    __device__ __forceinline__ V3 point() const { return get(a); }
+    __device__ __forceinline__ V3 normal() const { return get(n); }
+    __device__ __forceinline__ void set_point(V3 v) { put(a, v); }
+    __device__ __forceinline__ void set_normal(V3 v) { put(n, v); }
 };
+struct GlobalAnchors {
+    float *a, *n;
+    size_t S, i;
+    __device__ __forceinline__ GlobalAnchor at(int s) const { return GlobalAnchor{ a, n, S, i, s }; }
+    __device__ __forceinline__ GlobalAnchors fresh() const { return GlobalAnchors{ a, n, S, launder(i) }; }
+};
+
+/* substep-start poses of the block's worlds in LDS: [body * 7 + field][thread] */
+template <int BLOCK> struct LdsPoses {
+    float *p;
+    int tid;
+    __device__ __forceinline__ void store(int b, const Pose0 &v)
+    {
+        float *q = p + b * 7 * BLOCK + tid;
+        q[0] = v.x0.x; q[BLOCK] = v.x0.y; q[2 * BLOCK] = v.x0.z;
+        q[3 * BLOCK] = v.q0.w; q[4 * BLOCK] = v.q0.x; q[5 * BLOCK] = v.q0.y; q[6 * BLOCK] = v.q0.z;
+    }
+    __device__ __forceinline__ Pose0 load(int b) const
+    {
+        const float *q = p + b * 7 * BLOCK + tid;
+        Pose0 v;
+        v.x0 = v3(q[0], q[BLOCK], q[2 * BLOCK]);
+        v.q0 = q4(q[3 * BLOCK], q[4 * BLOCK], q[5 * BLOCK], q[6 * BLOCK]);
+        return v;
+    }
+};
+
+/* An index the optimiser cannot see through.  The world's SoA addresses are
+ * recomputed from it for the stores at the end of the step instead of being
+ * shared with the loads at its start, which would keep ~450 64-bit addresses
+ * live across the whole step. */
+__device__ __forceinline__ size_t opaque_index(size_t i)
+{
+    asm volatile("" : "+v"(i));
+    return i;
+}
 
 template <class TT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__restrict__ m, const TT *__restrict__ targets,
                                                 const Range *__restrict__ ranges, int n_ranges, int total)
 {
-    __shared__ V3 s_anc[NS * BLOCK], s_ancn[NS * BLOCK];
+    __shared__ float s_pose[NB * 7 * BLOCK];
     const Model &M = *m;
-    LdsAnchors<BLOCK> A;
-    A.a = s_anc;
-    A.n = s_ancn;
-    A.tid = threadIdx.x;
+    LdsPoses<BLOCK> P0;
+    P0.p = s_pose;
+    P0.tid = threadIdx.x;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         int i = t;
         if (ranges) {
@@ -140,10 +189,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
         if (d.unstable[i])
             continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
         load_world(d, i, W);
-        static_for<0, NS>([&](auto s) {
-            A.anc(s) = v3(d.anc[(s * 3 + 0) * S + i], d.anc[(s * 3 + 1) * S + i], d.anc[(s * 3 + 2) * S + i]);
-            A.ancn(s) = v3(d.ancn[(s * 3 + 0) * S + i], d.ancn[(s * 3 + 1) * S + i], d.ancn[(s * 3 + 2) * S + i]);
-        });
+        GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
         TerrainRef tr;
         tr.h = d.terrain + (size_t)i * d.cells;
         tr.outside = d.outside[i];
@@ -151,15 +197,11 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
         cs.base = d.cbuf;
         cs.stride = S;
         cs.world = i;
-        int ov = lane_step(W, A, M, tr, cs);
-        store_world(d, i, W);
-        static_for<0, NS>([&](auto s) {
-            V3 a = A.anc(s), n = A.ancn(s);
-            d.anc[(s * 3 + 0) * S + i] = a.x; d.anc[(s * 3 + 1) * S + i] = a.y; d.anc[(s * 3 + 2) * S + i] = a.z;
-            d.ancn[(s * 3 + 0) * S + i] = n.x; d.ancn[(s * 3 + 1) * S + i] = n.y; d.ancn[(s * 3 + 2) * S + i] = n.z;
-        });
+        int ov = lane_step(W, A, P0, M, tr, cs);
+        const size_t io = opaque_index((size_t)i);
+        store_world(d, io, W);
         if (ov)
-            d.err[i] |= ov;
+            d.err[io] |= ov;
     }
 }
 

@@ -35,6 +35,18 @@ constexpr int NCB = 9;
 constexpr int cbody(int i) { return i == 0 ? 0 : (i <= 4 ? 2 + 3 * (i - 1) : 3 + 3 * (i - 5)); }
 constexpr int cbody_index(int b) { return b == 0 ? 0 : ((b - 2) % 3 == 0 ? 1 + (b - 2) / 3 : 5 + (b - 3) / 3); }
 
+/* Values the optimiser cannot see through.  Addresses derived from them are
+ * recomputed where used instead of being hoisted out of the substep loop,
+ * which would keep hundreds of loop-invariant 64-bit addresses live. */
+XD size_t launder(size_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+template <class T> XD T *launder(T *p) { return (T *)launder((size_t)p); }
+
 template <int I, int N, class F> XD void static_for(F &&f)
 {
     if constexpr (I < N) {
@@ -59,8 +71,10 @@ struct Model {
 };
 
 /* Register-resident state of one world during a step.  The spheres'
- * friction anchors live in an AnchorStore (LDS on the device), and the
- * per-sphere contact counts of the step are packed four to a register. */
+ * friction anchors live in an anchor store (HBM on the device: they are
+ * touched only while a sphere is in contact), the substep-start poses in a
+ * pose store (LDS on the device), and the per-sphere contact counts of the
+ * step are packed four to a register. */
 constexpr int NTP = (NS + 3) / 4;
 struct LaneWorld {
     Body b[NB];
@@ -73,11 +87,16 @@ struct LaneWorld {
 };
 XD int touch_of(const LaneWorld &W, int s) { return (int)((W.touchp[s >> 2] >> (8 * (s & 3))) & 0xffu); }
 
-/* anchors of the host build: plain arrays */
+/* anchors and substep-start poses of the host build: plain arrays */
 struct ArrayAnchors {
     V3 a[NS], n[NS];
-    XD V3 &anc(int s) { return a[s]; }
-    XD V3 &ancn(int s) { return n[s]; }
+    XD AnchorRef at(int s) { return AnchorRef{ a[s], n[s] }; }
+    XD ArrayAnchors &fresh() { return *this; }
+};
+struct ArrayPoses {
+    Pose0 p[NB];
+    XD void store(int b, const Pose0 &v) { p[b] = v; }
+    XD Pose0 load(int b) const { return p[b]; }
 };
 
 /* Contact list storage: slot k field f of this world. */
@@ -102,8 +121,8 @@ struct ContactStore {
 /* One control step (arc_world_step, csrc/arc_phys.c:707-747).
  * Returns a bit mask of bodies whose contact list overflowed its slots
  * (never expected for the robot; reported as an error by the host). */
-template <class Anchors>
-XD int lane_step(LaneWorld &W, Anchors &A, const Model &M, const TerrainRef &tr, const ContactStore &cs)
+template <class Anchors, class Poses>
+XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const TerrainRef &tr0, const ContactStore &cs0)
 {
     int overflow = 0;
     /* prologue: refresh caches, clear counters (lim is precomputed per joint) */
@@ -112,7 +131,13 @@ XD int lane_step(LaneWorld &W, Anchors &A, const Model &M, const TerrainRef &tr,
     static_for<0, NJ>([&](auto ji) { W.tacc[ji] = 0.0f; });
 
     for (int sub = 0; sub < M.substeps; ++sub) {
-        static_for<0, NB>([&](auto bi) { integrate<body_kind(bi)>(W.b[bi], M.body[bi], M.g, M.h); });
+        /* per-substep copies of the memory handles, see launder() */
+        auto &&A = A0.fresh();
+        ContactStore cs = cs0;
+        cs.world = launder(cs0.world);
+        TerrainRef tr = tr0;
+        tr.h = launder(tr0.h);
+        static_for<0, NB>([&](auto bi) { P0.store(bi, integrate<body_kind(bi)>(W.b[bi], M.body[bi], M.g, M.h)); });
 
         /* joint positions, Gauss-Seidel in joint order */
         static_for<0, NJ>([&](auto ji) {
@@ -142,14 +167,15 @@ XD int lane_step(LaneWorld &W, Anchors &A, const Model &M, const TerrainRef &tr,
                     total++;
                 }
             };
+            auto an = A.at(si);
+            auto pose0 = [&]() { return P0.load(B); };
             collide<body_kind(B), sphere_at(si)>(W.b[B], M.body[B], M.sphere[si].at, M.sphere[si].rad,
-                                                 hit, anch, A.anc(si), A.ancn(si), M.terr, tr, rec,
-                                                 M.mus, M.mud);
+                                                 hit, anch, an, pose0, M.terr, tr, rec, M.mus, M.mud);
             W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
             W.touchp[si >> 2] += (uint32_t)hit << (8 * (si & 3));
         });
 
-        static_for<0, NB>([&](auto bi) { velocities(W.b[bi], M.inv_h, M.two_h); });
+        static_for<0, NB>([&](auto bi) { velocities(W.b[bi], P0.load(bi), M.inv_h, M.two_h); });
 
         static_for<0, NJ>([&](auto ji) {
             constexpr int P = joint_parent(ji), C = joint_child(ji);

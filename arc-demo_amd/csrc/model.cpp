@@ -50,8 +50,8 @@ int add_body(HostWorld &w, float mass, V3 inertia, V3 pos, Q4 q)
     Body &b = w.b[w.nb];
     BodyK &k = w.bk[w.nb];
     memset(&b, 0, sizeof(b));
-    b.x = b.x0 = pos;
-    b.q = b.q0 = qunit(q);
+    b.x = pos;
+    b.q = qunit(q);
     k.mass = mass;
     k.invm = 1.0f / mass;
     k.I = inertia;
@@ -122,8 +122,6 @@ void set_pose(HostWorld &w, V3 pos, Q4 q, const float *angles)
     }
     for (int i = 0; i < w.nb; ++i) {
         Body &b = w.b[i];
-        b.x0 = b.x;
-        b.q0 = b.q;
         b.v = zero3();
         b.w = zero3();
         sync_generic(b, w.bk[i]);

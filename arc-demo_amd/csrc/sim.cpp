@@ -436,7 +436,8 @@ int arcx_device_free(void *p) { return p ? (int)hipFree(p) : 0; }
 
 int arcx_memcpy(void *dst, const void *src, size_t bytes, int to_device)
 {
-    hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+    hipMemcpyKind k = to_device == 1 ? hipMemcpyHostToDevice : to_device == 2 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    hipError_t e = hipMemcpy(dst, src, bytes, k);
     return (int)e;
 }
 

@@ -30,6 +30,7 @@ struct WaveWorld {
     float angle[WAVE_MAX_JOINTS], rate[WAVE_MAX_JOINTS], torque[WAVE_MAX_JOINTS];
     int touch[WAVE_MAX_SPHERES], anchored[WAVE_MAX_SPHERES];
     V3 anc[WAVE_MAX_SPHERES], ancn[WAVE_MAX_SPHERES];
+    Pose0 p0[WAVE_MAX_BODIES];
     float cl[WAVE_MAX_SLOTS][6];
     int ccount[WAVE_MAX_BODIES];
     int unstable;
@@ -84,8 +85,10 @@ __device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M,
                 total++;
             }
         };
-        collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], W.anc[s],
-                              W.ancn[s], M.terr, tr, rec, M.mus, M.mud);
+        AnchorRef an{ W.anc[s], W.ancn[s] };
+        auto pose0 = [&]() { return W.p0[B]; };
+        collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], an, pose0,
+                              M.terr, tr, rec, M.mus, M.mud);
     }
     W.b[B] = b;
 }
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
         for (int sub = 0; sub < M.substeps; ++sub) {
             if (lane < T.nb) {
                 Body b = W.b[lane];
-                integrate<KGEN>(b, M.body[lane], M.g, M.h);
+                W.p0[lane] = integrate<KGEN>(b, M.body[lane], M.g, M.h);
                 W.b[lane] = b;
                 W.ccount[lane] = 0;
             }
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
             __syncthreads();
             if (lane < T.nb) {
                 Body b = W.b[lane];
-                velocities(b, M.inv_h, M.two_h);
+                velocities(b, W.p0[lane], M.inv_h, M.two_h);
                 W.b[lane] = b;
             }
             __syncthreads();

@@ -146,11 +146,19 @@ struct BodyK {
 
 /* per-body dynamic state of one world */
 struct Body {
-    V3 x, x0;
-    Q4 q, q0;
+    V3 x;
+    Q4 q;
     V3 v, w;
     M3 R;   /* rotation of q / |q| */
     M3 Iw;  /* world inverse inertia (KANI / KGEN only) */
+};
+
+/* a body's pose at the start of the current substep (x_prev, q_prev of the
+ * reference), written by integrate and read by the velocity update and by
+ * friction touch-downs; kept outside Body so callers can store it apart */
+struct Pose0 {
+    V3 x0;
+    Q4 q0;
 };
 
 template <int KIND> XD bool iso_inv(const BodyK &k) { return KIND == KISO ? true : KIND == KANI ? false : k.iso_invI != 0; }
@@ -460,18 +468,27 @@ template <int AT> XD V3 at_world(const Body &b, V3 at)
     return add(b.x, brot(b, at));
 }
 /* x0 + q0 (at), the sphere centre at the substep start */
-template <int AT> XD V3 at_world0(const Body &b, V3 at)
+template <int AT> XD V3 at_world0(const Pose0 &p, V3 at)
 {
     if (AT == AT_ZERO)
-        return b.x0;
+        return p.x0;
     if (AT == AT_Z) {
-        const Q4 q = b.q0;
+        const Q4 q = p.q0;
         float z = at.z;
         float tx = (q.y * z) * 2.0f, ty = (-(q.x * z)) * 2.0f;
-        return add(b.x0, v3(tx * q.w - q.z * ty, ty * q.w + q.z * tx, z + (q.x * ty - q.y * tx)));
+        return add(p.x0, v3(tx * q.w - q.z * ty, ty * q.w + q.z * tx, z + (q.x * ty - q.y * tx)));
     }
-    return add(b.x0, qrot(b.q0, at));
+    return add(p.x0, qrot(p.q0, at));
 }
+
+/* friction anchor of one sphere held in plain variables */
+struct AnchorRef {
+    V3 &a, &n;
+    XD V3 point() const { return a; }
+    XD V3 normal() const { return n; }
+    XD void set_point(V3 v) { a = v; }
+    XD void set_normal(V3 v) { n = v; }
+};
 
 struct SphereK {
     V3 at;
@@ -504,19 +521,26 @@ XD float terrain_height(const TerrainK &t, const TerrainRef &r, float x, float y
     return r.h[(int)fx + (int)fy * t.nx];
 }
 
-/* anchored static / kinetic friction, arc_phys.c:512-539 */
-template <int KIND, int AT>
-XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, V3 &anc, V3 &ancn,
+/* anchored static / kinetic friction, arc_phys.c:512-539.  The anchor is
+ * read and written through `an` (only when the sphere touches), the
+ * substep-start pose fetched through pose0() only at a touch-down. */
+template <int KIND, int AT, class Anchor, class PoseFn>
+XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Anchor &an, const PoseFn &pose0,
                  V3 n, float ln, float mus, float mud)
 {
     V3 c = at_world<AT>(b, at);
     V3 p = mad(c, n, -rad);
     float frac = 1.0f;
-    if (!anchored || dot(n, ancn) < 0.9f) {
-        V3 c0 = at_world0<AT>(b, at);
+    V3 anc;
+    if (!anchored || dot(n, an.normal()) < 0.9f) {
+        /* touch down: anchor where the contact point was at the substep start */
+        V3 c0 = at_world0<AT>(pose0(), at);
         anc = mad(c0, n, -rad);
-        ancn = n;
+        an.set_point(anc);
+        an.set_normal(n);
         anchored = 1;
+    } else {
+        anc = an.point();
     }
     V3 e = sub(p, anc);
     e = mad(e, n, -dot(e, n));
@@ -528,7 +552,7 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, V3 &a
     float lt = l / wt;
     if (lt > mus * ln) {
         frac = minf_(mud * ln / lt, 1.0f);
-        anc = mad(anc, e, 1.0f - frac);
+        an.set_point(mad(anc, e, 1.0f - frac));
     }
     pos_static<KIND>(b, k, r, scl(e, -frac));
 }
@@ -536,9 +560,9 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, V3 &a
 /* One sphere against the column grid, each contact solved at once and
  * recorded through rec(local, n) for the velocity pass; arc_phys.c:542-610.
  * Returns nothing; updates touch / anchored / anchor state. */
-template <int KIND, int AT, class Rec>
-XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anchored, V3 &anc,
-                V3 &ancn, const TerrainK &t, const TerrainRef &tr, Rec &rec, float mus, float mud)
+template <int KIND, int AT, class Rec, class Anchor, class PoseFn>
+XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anchored, Anchor &an,
+                const PoseFn &pose0, const TerrainK &t, const TerrainRef &tr, Rec &rec, float mus, float mud)
 {
     float ln = 0.0f, dmax = 0.0f;
     V3 c = at_world<AT>(b, at), nmax = v3(0.0f, 0.0f, 1.0f);
@@ -593,7 +617,7 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
         }
     }
     if (hit)
-        friction<KIND, AT>(b, k, at, rad, anchored, anc, ancn, nmax, ln, mus, mud);
+        friction<KIND, AT>(b, k, at, rad, anchored, an, pose0, nmax, ln, mus, mud);
     else
         anchored = 0;
     touch += hit;
@@ -640,26 +664,28 @@ XD V3 gyro(const BodyK &k, V3 w0, float h)
     return sub(w0, dw);
 }
 
-/* arc_phys.c:658-672 */
+/* arc_phys.c:658-672; returns the substep-start pose */
 template <int KIND>
-XD void integrate(Body &b, const BodyK &k, V3 g, float h)
+XD Pose0 integrate(Body &b, const BodyK &k, V3 g, float h)
 {
-    b.x0 = b.x;
-    b.q0 = b.q;
+    Pose0 p0;
+    p0.x0 = b.x;
+    p0.q0 = b.q;
     b.v = mad(b.v, g, h);
     b.x = mad(b.x, b.v, h);
     if (!iso_in<KIND>(k))
         b.w = mmul(b.R, gyro(k, mtmul(b.R, b.w), h));
     b.q = qturn(b.q, scl(b.w, h));
     bsync<KIND>(b, k);
+    return p0;
 }
 
 /* arc_phys.c:674-687; R and Iw are deliberately not refreshed here */
-XD void velocities(Body &b, float inv_h, float two_h)
+XD void velocities(Body &b, const Pose0 &p0, float inv_h, float two_h)
 {
     b.q = qunit(b.q);
-    Q4 dq = qmul(b.q, conj(b.q0));
-    b.v = scl(sub(b.x, b.x0), inv_h);
+    Q4 dq = qmul(b.q, conj(p0.q0));
+    b.v = scl(sub(b.x, p0.x0), inv_h);
     b.w = scl(v3(dq.x, dq.y, dq.z), two_h);
     if (dq.w < 0.0f)
         b.w = neg(b.w);

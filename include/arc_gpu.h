@@ -31,7 +31,8 @@ void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out);
 
 /* Plain device memory for inputs kept on the GPU (synthetic targets,
  * client buffers): hipMalloc / hipFree / synchronous copies.  to_device = 1
- * copies host -> device, 0 device -> host.  Returns NULL / a hipError_t. */
+ * copies host -> device, 0 device -> host, 2 device -> device.  Returns NULL
+ * / a hipError_t. */
 void *arcx_device_alloc(size_t bytes);
 int arcx_device_free(void *p);
 int arcx_memcpy(void *dst, const void *src, size_t bytes, int to_device);

@@ -122,7 +122,8 @@ void hc_step(HC *h, const double *targets)
         cs.base = h->cbuf.data();
         cs.stride = 1;
         cs.world = 0;
-        h->err[i] |= lane_step(W, h->anc[i], M, tr, cs);
+        ArrayPoses poses;
+        h->err[i] |= lane_step(W, h->anc[i], poses, M, tr, cs);
     }
 }
 
