@@ -22,7 +22,7 @@ OBJ = PKG / "_build"
 INCLUDE = PKG.parent / "include"
 
 ARCH = os.environ.get("ARC_OFFLOAD_ARCH", "gfx950")
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", f"-I{INCLUDE}", f"-I{CSRC}",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", f"-I{INCLUDE}", f"-I{CSRC}", f"-I{OBJ}",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-fno-gpu-rdc"]
@@ -53,7 +53,21 @@ def compile_one(src: str, device: bool) -> Path:
     return obj
 
 
+def generate_constants() -> Path:
+    """Compiles and runs gen_robot.cpp: the robot's constants as literals."""
+    OBJ.mkdir(exist_ok=True)
+    exe = OBJ / "gen_robot"
+    _run([hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-fno-math-errno", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+          "-I/opt/rocm/include", f"-I{CSRC}", str(CSRC / "gen_robot.cpp"), str(CSRC / "model.cpp"), "-o", str(exe)])
+    r = subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+    out = OBJ / "robot_consts.h"
+    if not out.exists() or out.read_text() != r.stdout:
+        out.write_text(r.stdout)
+    return out
+
+
 def build(jobs: int = 4) -> Path:
+    generate_constants()
     tasks = [(s, True) for s in HIP_SOURCES] + [(s, False) for s in HOST_SOURCES]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda t: compile_one(*t), tasks))

@@ -17,6 +17,7 @@
 #pragma once
 #include <type_traits>
 #include "xpbd_core.h"
+#include "robot_consts.h"  /* generated at build time by gen_robot.cpp */
 
 namespace xpbd {
 
@@ -54,6 +55,16 @@ template <int I, int N, class F> XD void static_for(F &&f)
         static_for<I + 1, N>(f);
     }
 }
+
+static_assert(robot::NBODY == NB && robot::NJOINT == NJ && robot::NSPHERE == NS, "robot topology");
+
+/* joint constants as the kernel sees them: geometry and gains compiled in
+ * (robot::joint), the three products with the substep length from the Model */
+#define ROBOT_JOINT(ji, M)                            \
+    JointK J = robot::joint[ji];                      \
+    J.soft = (M).joint[ji].soft;                      \
+    J.cap_pos = (M).joint[ji].cap_pos;                \
+    J.tmax_h = (M).joint[ji].tmax_h
 
 struct Model {
     BodyK body[NB];
@@ -126,7 +137,7 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
 {
     int overflow = 0;
     /* prologue: refresh caches, clear counters (lim is precomputed per joint) */
-    static_for<0, NB>([&](auto bi) { bsync<body_kind(bi)>(W.b[bi], M.body[bi]); });
+    static_for<0, NB>([&](auto bi) { bsync<body_kind(bi)>(W.b[bi], robot::body[bi]); });
     static_for<0, NTP>([&](auto k) { W.touchp[k] = 0u; });
     static_for<0, NJ>([&](auto ji) { W.tacc[ji] = 0.0f; });
 
@@ -137,12 +148,13 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
         cs.world = launder(cs0.world);
         TerrainRef tr = tr0;
         tr.h = launder(tr0.h);
-        static_for<0, NB>([&](auto bi) { P0.store(bi, integrate<body_kind(bi)>(W.b[bi], M.body[bi], M.g, M.h)); });
+        static_for<0, NB>([&](auto bi) { P0.store(bi, integrate<body_kind(bi)>(W.b[bi], robot::body[bi], M.g, M.h)); });
 
         /* joint positions, Gauss-Seidel in joint order */
         static_for<0, NJ>([&](auto ji) {
             constexpr int P = joint_parent(ji), C = joint_child(ji);
-            joint_positions<body_kind(P), body_kind(C)>(&W.b[P], &M.body[P], W.b[C], M.body[C], M.joint[ji],
+            ROBOT_JOINT(ji, M);
+            joint_positions<body_kind(P), body_kind(C)>(&W.b[P], &robot::body[P], W.b[C], robot::body[C], J,
                                                         W.target[ji], M.hh, W.tacc[ji]);
         });
 
@@ -169,8 +181,9 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
             };
             auto an = A.at(si);
             auto pose0 = [&]() { return P0.load(B); };
-            collide<body_kind(B), sphere_at(si)>(W.b[B], M.body[B], M.sphere[si].at, M.sphere[si].rad,
-                                                 hit, anch, an, pose0, M.terr, tr, rec, M.mus, M.mud);
+            collide<body_kind(B), sphere_at(si)>(W.b[B], robot::body[B], robot::sphere[si].at,
+                                                 robot::sphere[si].rad, hit, anch, an, pose0, M.terr, tr, rec,
+                                                 M.mus, M.mud);
             W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
             W.touchp[si >> 2] += (uint32_t)hit << (8 * (si & 3));
         });
@@ -179,8 +192,9 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
 
         static_for<0, NJ>([&](auto ji) {
             constexpr int P = joint_parent(ji), C = joint_child(ji);
-            joint_velocities<body_kind(P), body_kind(C)>(&W.b[P], &M.body[P], W.b[C], M.body[C], M.joint[ji],
-                                                         M.h, W.tacc[ji]);
+            ROBOT_JOINT(ji, M);
+            joint_velocities<body_kind(P), body_kind(C)>(&W.b[P], &robot::body[P], W.b[C], robot::body[C], J, M.h,
+                                                         W.tacc[ji]);
         });
 
         static_for<0, NCB>([&](auto ci) {
@@ -189,7 +203,7 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
             for (int k = 0; k < n; ++k) {
                 V3 l, nn;
                 cs.get(M.cslot[ci] + k, l, nn);
-                contact_velocity<body_kind(B)>(W.b[B], M.body[B], l, nn);
+                contact_velocity<body_kind(B)>(W.b[B], robot::body[B], l, nn);
             }
         });
     }
@@ -197,8 +211,8 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
     /* epilogue: joint outputs, time, stability */
     static_for<0, NJ>([&](auto ji) {
         constexpr int P = joint_parent(ji), C = joint_child(ji);
-        W.angle[ji] = joint_angle<true>(&W.b[P], W.b[C], M.joint[ji]);
-        W.rate[ji] = joint_rate<true>(&W.b[P], W.b[C], M.joint[ji]);
+        W.angle[ji] = joint_angle<true>(&W.b[P], W.b[C], robot::joint[ji]);
+        W.rate[ji] = joint_rate<true>(&W.b[P], W.b[C], robot::joint[ji]);
         W.torque[ji] = W.tacc[ji] / (float)M.substeps;
     });
     W.time += M.dt;

@@ -35,6 +35,30 @@ static bool topology_ok(const HostWorld &w)
     return true;
 }
 
+static bool same_bits(const void *a, const void *b, size_t n) { return memcmp(a, b, n) == 0; }
+
+/* The step kernel compiles the robot's constants in (robot_consts.h); they must
+ * be the bits of the model built here. */
+static bool constants_match(const HostWorld &w)
+{
+    for (int b = 0; b < NB; ++b)
+        if (!same_bits(&robot::body[b], &w.bk[b], sizeof(BodyK)))
+            return false;
+    for (int j = 0; j < NJ; ++j) {
+        JointK a = robot::joint[j], k = joint_constants(w.j[j], 1.0f);
+        a.soft = k.soft = a.cap_pos = k.cap_pos = a.tmax_h = k.tmax_h = 0.0f;
+        if (!same_bits(&a, &k, sizeof(JointK)))
+            return false;
+    }
+    for (int s = 0; s < NS; ++s) {
+        const SphereK &k = robot::sphere[s];
+        if (!same_bits(&k.at, &w.s[s].at, sizeof(V3)) || !same_bits(&k.rad, &w.s[s].rad, sizeof(float))
+            || k.body != w.s[s].body)
+            return false;
+    }
+    return true;
+}
+
 bool sim_setup(SimSetup &S, double dt, int substeps, int grid, double cell)
 {
     if (substeps <= 0 || grid <= 0 || !(cell > 0.0) || !(dt > 0.0)) {
@@ -47,6 +71,10 @@ bool sim_setup(SimSetup &S, double dt, int substeps, int grid, double cell)
     terrain_flat(S.tmpl, S.grid, S.grid, (float)cell, 0.0f);
     if (!topology_ok(S.tmpl)) {
         fprintf(stderr, "libarcsim: robot topology does not match the device engine\n");
+        return false;
+    }
+    if (!constants_match(S.tmpl)) {
+        fprintf(stderr, "libarcsim: compiled-in robot constants differ from the built robot (rebuild)\n");
         return false;
     }
     for (int i = 0; i < S.tmpl.ns; ++i)
