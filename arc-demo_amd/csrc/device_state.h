@@ -25,6 +25,8 @@ struct DevWorlds {
     int n;            /* worlds */
     size_t stride;    /* SoA stride (n rounded up to a multiple of 64) */
     int cells;        /* nx * ny terrain columns per world */
+    int tpitch;       /* nx + 2 * TPAD: row pitch of the stored terrain */
+    int tcells;       /* tpitch^2 bytes of stored terrain per world */
     float *x;         /* [NB*3][stride] body position */
     float *q;         /* [NB*4][stride] body orientation w,x,y,z */
     float *v;         /* [NB*3][stride] linear velocity */
@@ -37,7 +39,8 @@ struct DevWorlds {
     float *ancn;      /* [NS*3][stride] anchor normal */
     int *unstable;    /* [stride] */
     float *time;      /* [stride] */
-    float *terrain;   /* [n][cells] column heights */
+    int8_t *tq;       /* [n][tcells] column steps (TerrainRef), world-major */
+    float *tbase;     /* [stride] terrain base (TerrainRef) */
     float *outside;   /* [stride] height outside the grid */
     float *difficulty; /* [stride] terrain difficulty of the last reset */
     float *cbuf;      /* [cslots*6][stride] per-substep contact list */
@@ -86,8 +89,18 @@ struct Range {
 };
 
 /* Kernel launchers (kernels.hip).  All are asynchronous on `stream`. */
-size_t dev_bytes_per_world(int cells, int cslots);
-hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots);
+size_t dev_bytes_per_world(int grid, int cslots);
+hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots);
+/* world i's terrain view */
+__host__ __device__ inline TerrainRef terrain_ref(const DevWorlds &d, size_t i)
+{
+    TerrainRef r;
+    r.q = d.tq + i * (size_t)d.tcells + TPAD + TPAD * d.tpitch;
+    r.pitch = d.tpitch;
+    r.base = 0.0f;
+    r.outside = 0.0f;
+    return r;
+}
 void dev_free(DevWorlds &d);
 hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, float outside, hipStream_t s);
 /* targets: n*NJ world-major, double (host ABI) or float (device inputs) */

@@ -19,20 +19,24 @@ static inline size_t round64(size_t n) { return (n + 63) & ~(size_t)63; }
 /* float counts per world in the SoA block (terrain and contact list apart) */
 static const int SOA_FLOATS = NB * 3 + NB * 4 + NB * 3 + NB * 3 + NJ + 3 * NJ + NS * 3 + NS * 3 + 1 + 1 + 1;
 
-size_t dev_bytes_per_world(int cells, int cslots)
+static size_t tcells_of(int grid) { return (size_t)(grid + 2 * TPAD) * (grid + 2 * TPAD); }
+
+size_t dev_bytes_per_world(int grid, int cslots)
 {
-    return sizeof(float) * ((size_t)SOA_FLOATS + cells + 1 + (size_t)cslots * 6) + NS /* touch */
+    return sizeof(float) * ((size_t)SOA_FLOATS + 2 + (size_t)cslots * 6) + tcells_of(grid) + NS /* touch */
            + sizeof(uint32_t) + 2 * sizeof(int);
 }
 
-hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots)
+hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
 {
     d.n = n;
     d.stride = round64((size_t)(n > 0 ? n : 1));
-    d.cells = cells;
+    d.cells = grid * grid;
+    d.tpitch = grid + 2 * TPAD;
+    d.tcells = (int)tcells_of(grid);
     size_t S = d.stride;
-    size_t floats = S * ((size_t)SOA_FLOATS + 1 + (size_t)cslots * 6) + (size_t)S * cells;
-    size_t bytes = floats * sizeof(float) + S * NS + S * sizeof(uint32_t) + 2 * S * sizeof(int) + 1024;
+    size_t floats = S * ((size_t)SOA_FLOATS + 2 + (size_t)cslots * 6);
+    size_t bytes = floats * sizeof(float) + S * (size_t)d.tcells + S * NS + S * sizeof(uint32_t) + 2 * S * sizeof(int) + 1024;
     char *p = nullptr;
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e != hipSuccess)
@@ -49,12 +53,13 @@ hipError_t dev_alloc(DevWorlds &d, int n, int cells, int cslots)
     d.time = f; f += S;
     d.outside = f; f += S;
     d.difficulty = f; f += S;
+    d.tbase = f; f += S;
     d.cbuf = f; f += (size_t)cslots * 6 * S;
-    d.terrain = f; f += S * cells;
     d.unstable = (int *)f;
     d.err = d.unstable + S;
     d.anchored = (uint32_t *)(d.err + S);
     d.touch = (uint8_t *)(d.anchored + S);
+    d.tq = (int8_t *)(d.touch + S * NS);
     return hipSuccess;
 }
 
@@ -190,8 +195,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
             continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
         load_world(d, i, W);
         GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
-        TerrainRef tr;
-        tr.h = d.terrain + (size_t)i * d.cells;
+        TerrainRef tr = terrain_ref(d, i);
+        tr.base = d.tbase[i];
         tr.outside = d.outside[i];
         ContactStore cs;
         cs.base = d.cbuf;
@@ -243,6 +248,7 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
                                               int n_ids, const int *__restrict__ count_dev)
 {
     __shared__ float hbuf[GRID_LIMIT_CELLS];
+    __shared__ int8_t qbuf[(64 + 2 * TPAD) * (64 + 2 * TPAD)];
     __shared__ float lat[LAT_LIMIT];
     __shared__ float draws[NJ + 1];
     const Model &M = *m;
@@ -255,26 +261,35 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
     const int n = R.grid, cells = n * n;
     const bool gen = dfc > 0.0f;
     const size_t S = d.stride;
+    const int pitch = d.tpitch;
 
     /* the draws are one sequential stream: lane 0 produces them */
     if (tid == 0)
         reset_draws(seeds[k], gen, n, lat, hbuf, draws);
     __syncthreads();
+    /* quantisation steps of the columns (csrc/arc_terrain.c:54-71) */
     for (int c = tid; c < cells; c += 64)
-        hbuf[c] = gen ? column_height(lat, n, dfc, hbuf[c], c) : 0.0f;
+        hbuf[c] = gen ? column_step(lat, n, dfc, hbuf[c], c) : 0.0f;
+    __syncthreads();
+    /* base: the pre-shift height of the column at the origin */
+    float base = 0.0f;
+    if (gen) {
+        int ix = (int)floorf((0.0f - M.terr.ox) / M.terr.cell), iy = (int)floorf((0.0f - M.terr.oy) / M.terr.cell);
+        base = 0.03f * hbuf[ix + iy * n];
+    }
+    int8_t *gq = d.tq + (size_t)i * d.tcells;
+    for (int c = tid; c < d.tcells; c += 64) {
+        int ix = c % pitch - TPAD, iy = c / pitch - TPAD;
+        int8_t v = (ix >= 0 && iy >= 0 && ix < n && iy < n) ? (int8_t)(int)hbuf[ix + iy * n] : (int8_t)Q_OUT;
+        qbuf[c] = v;
+        gq[c] = v;
+    }
     __syncthreads();
     TerrainRef tl;
-    tl.h = hbuf;
+    tl.q = qbuf + TPAD + TPAD * pitch;
+    tl.pitch = pitch;
+    tl.base = base;
     tl.outside = -100.0f;
-    if (gen) { /* shift so that the column at the origin is at z = 0 */
-        float base = terrain_height(M.terr, tl, 0.0f, 0.0f);
-        __syncthreads();
-        for (int c = tid; c < cells; c += 64)
-            hbuf[c] -= base;
-        __syncthreads();
-    }
-    for (int c = tid; c < cells; c += 64)
-        d.terrain[(size_t)i * d.cells + c] = hbuf[c];
 
     if (tid == 0) {
         float ang[NJ];
@@ -310,6 +325,7 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
         d.outside[i] = -100.0f;
+        d.tbase[i] = base;
         d.difficulty[i] = dfc;
         d.err[i] = 0;
     }
@@ -426,10 +442,21 @@ __global__ void k_init_template(DevWorlds d, const TemplateK *__restrict__ tk, f
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
         d.outside[i] = outside;
+        d.tbase[i] = 0.0f;
         d.difficulty[i] = 0.0f;
         d.err[i] = 0;
-        for (int c = 0; c < d.cells; ++c)
-            d.terrain[i * d.cells + c] = 0.0f;
+    }
+}
+
+/* flat template terrain: steps 0 inside the grid, the border outside */
+__global__ void k_init_terrain(DevWorlds d)
+{
+    const int n = d.tpitch - 2 * TPAD;
+    size_t total = (size_t)d.n * d.tcells;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        int c = (int)(t % d.tcells);
+        int ix = c % d.tpitch - TPAD, iy = c / d.tpitch - TPAD;
+        d.tq[t] = (ix >= 0 && iy >= 0 && ix < n && iy < n) ? (int8_t)0 : (int8_t)Q_OUT;
     }
 }
 
@@ -439,6 +466,7 @@ hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, flo
     if (blocks < 1)
         blocks = 1;
     hipLaunchKernelGGL(k_init_template, dim3(blocks), dim3(256), 0, s, d, tk_dev, outside);
+    hipLaunchKernelGGL(k_init_terrain, dim3(4096), dim3(256), 0, s, d);
     return hipGetLastError();
 }
 
@@ -510,8 +538,8 @@ __global__ void k_heights(DevWorlds d, const Model *__restrict__ m, const double
     float yaw = f_atan2(fwd.y, fwd.x), c = f_cos(yaw), sn = f_sin(yaw);
     float f = (float)offsets[2 * j], l = (float)offsets[2 * j + 1];
     float px = d.x[0 * S + i], py = d.x[1 * S + i];
-    TerrainRef tr;
-    tr.h = d.terrain + i * d.cells;
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
     tr.outside = d.outside[i];
     out[t] = terrain_height(M.terr, tr, px + c * f - sn * l, py + sn * f + c * l);
 }
@@ -533,8 +561,8 @@ __global__ void k_terrain_at(DevWorlds d, const Model *__restrict__ m, const int
     if (t >= mcount)
         return;
     int i = ids[t];
-    TerrainRef tr;
-    tr.h = d.terrain + (size_t)i * d.cells;
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
     tr.outside = d.outside[i];
     out[t] = terrain_height(m->terr, tr, (float)xy[2 * t], (float)xy[2 * t + 1]);
 }

@@ -147,7 +147,7 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
         ContactStore cs = cs0;
         cs.world = launder(cs0.world);
         TerrainRef tr = tr0;
-        tr.h = launder(tr0.h);
+        tr.q = launder(tr0.q);
         static_for<0, NB>([&](auto bi) { P0.store(bi, integrate<body_kind(bi)>(W.b[bi], robot::body[bi], M.g, M.h)); });
 
         /* joint positions, Gauss-Seidel in joint order */
@@ -164,8 +164,17 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
         int ncount[NCB];
         int total = 0;
         static_for<0, NCB>([&](auto ci) { ncount[ci] = 0; });
+        /* each sphere's columns are fetched while the previous sphere is solved */
+        constexpr int B0 = sphere_body(0);
+        TBlock next = fetch_block(M.terr, tr, at_world<sphere_at(0)>(W.b[B0], robot::sphere[0].at), robot::sphere[0].rad);
         static_for<0, NS>([&](auto si) {
             constexpr int B = sphere_body(si);
+            const TBlock blk = next;
+            if constexpr (si + 1 < NS) {
+                constexpr int BN = sphere_body(si + 1);
+                next = fetch_block(M.terr, tr, at_world<sphere_at(si + 1)>(W.b[BN], robot::sphere[si + 1].at),
+                                   robot::sphere[si + 1].rad);
+            }
             constexpr int CI = cbody_index(B);
             int anch = (W.anchored >> si) & 1u;
             int hit = 0;
@@ -182,7 +191,7 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
             auto an = A.at(si);
             auto pose0 = [&]() { return P0.load(B); };
             collide<body_kind(B), sphere_at(si)>(W.b[B], robot::body[B], robot::sphere[si].at,
-                                                 robot::sphere[si].rad, hit, anch, an, pose0, M.terr, tr, rec,
+                                                 robot::sphere[si].rad, hit, anch, an, pose0, M.terr, tr, blk, rec,
                                                  M.mus, M.mud);
             W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
             W.touchp[si >> 2] += (uint32_t)hit << (8 * (si & 3));

@@ -75,8 +75,10 @@ XD void reset_draws(uint32_t seed, bool gen, int n, float *lat, float *jit, floa
     draws[NJ] = r.range(-3.14159265358979323846f, 3.14159265358979323846f);
 }
 
-/* height of column c before the re-basing: hills + bumps + jitter, quantised to 3 cm */
-XD float column_height(const float *lat, int n, float difficulty, float jitter, int c)
+/* quantisation step F of column c before the re-basing: hills + bumps +
+ * jitter, h = 0.03f * F with F = floor(h / 0.03 + 0.5) (F is small: |F| <= 21 at
+ * difficulty 1) */
+XD float column_step(const float *lat, int n, float difficulty, float jitter, int c)
 {
     const int m1 = lattice1(n), m2 = lattice2(n);
     int ix = c % n, iy = c / n;
@@ -84,7 +86,7 @@ XD float column_height(const float *lat, int n, float difficulty, float jitter, 
     h = h + (0.45f * difficulty) * noise_at(lat, m1, 8, ix, iy);
     h = h + (0.12f * difficulty) * noise_at(lat + m1 * m1, m2, 3, ix, iy);
     h = h + 0.06f * difficulty * jitter;
-    return 0.03f * floorf(h / 0.03f + 0.5f);
+    return floorf(h / 0.03f + 0.5f);
 }
 
 /* highest column under the robot's footprint, csrc/arc_sim.c:104-107 */

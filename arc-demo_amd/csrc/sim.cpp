@@ -180,7 +180,7 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
     }
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess
-        || (e = dev_alloc(s->dev, n_worlds, s->su.grid * s->su.grid, M.cslots)) != hipSuccess
+        || (e = dev_alloc(s->dev, n_worlds, s->su.grid, M.cslots)) != hipSuccess
         || (e = hipMalloc((void **)&s->d_model, sizeof(Model))) != hipSuccess
         || (e = hipMalloc((void **)&s->d_reset, sizeof(ResetK))) != hipSuccess
         || (e = hipMalloc((void **)&s->d_tmpl, sizeof(TemplateK))) != hipSuccess
@@ -483,15 +483,24 @@ int arcx_terrain(const arc_sim *cs, int i, double *out)
     arc_sim *s = const_cast<arc_sim *>(cs);
     if (i < 0 || i >= s->n)
         return -1;
-    int cells = s->dev.cells;
-    std::vector<float> buf(cells + 1);
-    HIPCHK_RET(s, hipMemcpyAsync(buf.data(), s->dev.terrain + (size_t)i * cells, cells * sizeof(float),
-                                 hipMemcpyDeviceToHost, s->stream), "arcx_terrain", -1);
-    HIPCHK_RET(s, hipMemcpyAsync(buf.data() + cells, s->dev.outside + i, sizeof(float), hipMemcpyDeviceToHost, s->stream),
+    const DevWorlds &d = s->dev;
+    std::vector<int8_t> q(d.tcells);
+    float bo[2];
+    HIPCHK_RET(s, hipMemcpyAsync(q.data(), d.tq + (size_t)i * d.tcells, d.tcells, hipMemcpyDeviceToHost, s->stream),
                "arcx_terrain", -1);
+    HIPCHK_RET(s, hipMemcpyAsync(&bo[0], d.tbase + i, sizeof(float), hipMemcpyDeviceToHost, s->stream), "arcx_terrain", -1);
+    HIPCHK_RET(s, hipMemcpyAsync(&bo[1], d.outside + i, sizeof(float), hipMemcpyDeviceToHost, s->stream), "arcx_terrain", -1);
     HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arcx_terrain", -1);
-    for (int k = 0; k <= cells; ++k)
-        out[k] = buf[k];
+    TerrainRef r;
+    r.q = q.data() + TPAD + TPAD * d.tpitch;
+    r.pitch = d.tpitch;
+    r.base = bo[0];
+    r.outside = bo[1];
+    const int n = s->su.grid;
+    for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix)
+            out[ix + iy * n] = terrain_cell(s->su.model.terr, r, ix, iy);
+    out[n * n] = bo[1];
     return 0;
 }
 
@@ -618,7 +627,7 @@ int arcx_set_launch(arc_sim *s, int block, int grid_blocks)
     return 0;
 }
 
-double arcx_bytes_per_world(const arc_sim *s) { return (double)dev_bytes_per_world(s->dev.cells, s->su.model.cslots); }
+double arcx_bytes_per_world(const arc_sim *s) { return (double)dev_bytes_per_world(s->su.grid, s->su.model.cslots); }
 
 int arcx_world_error(const arc_sim *cs, int i)
 {

@@ -87,8 +87,10 @@ __device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M,
         };
         AnchorRef an{ W.anc[s], W.ancn[s] };
         auto pose0 = [&]() { return W.p0[B]; };
+        TBlock none;
+        none.ok = 0;
         collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], an, pose0,
-                              M.terr, tr, rec, M.mus, M.mud);
+                              M.terr, tr, none, rec, M.mus, M.mud);
     }
     W.b[B] = b;
 }
@@ -140,8 +142,8 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
             W.tacc[lane] = 0.0f;
         __syncthreads();
 
-        TerrainRef tr;
-        tr.h = d.terrain + (size_t)i * d.cells;
+        TerrainRef tr = terrain_ref(d, i);
+        tr.base = d.tbase[i];
         tr.outside = d.outside[i];
 
         for (int sub = 0; sub < M.substeps; ++sub) {

@@ -496,21 +496,29 @@ struct SphereK {
     int body;
 };
 
-/* Per-world block terrain: heights of nx * ny columns at `h` (row stride nx)
- * and the height outside the grid. */
+/* Per-world block terrain.  Every height the engine produces is a whole
+ * number F of 3 cm steps shifted by the world's base (csrc/arc_terrain.c:62-70:
+ * h = 0.03f * F, then h -= base), so a column is stored as F in one byte and
+ * read back as 0.03f * F - base, bit for bit the same binary32 value.  The grid
+ * carries a border of TPAD columns marked Q_OUT on every side; columns outside
+ * the grid read as the world's `outside` height. */
+constexpr int TPAD = 2;
+constexpr int Q_OUT = -128;
 struct TerrainRef {
-    const float *h;
-    float outside;
+    const int8_t *q;  /* column (0, 0); row pitch `pitch` */
+    int pitch;        /* nx + 2 * TPAD */
+    float base, outside;
 };
 struct TerrainK {
     int nx, ny;
     float cell, ox, oy;
 };
+XD float dequant(const TerrainRef &r, int k) { return k == Q_OUT ? r.outside : 0.03f * (float)k - r.base; }
 XD float terrain_cell(const TerrainK &t, const TerrainRef &r, int ix, int iy)
 {
     if (ix < 0 || iy < 0 || ix >= t.nx || iy >= t.ny)
         return r.outside;
-    return r.h[ix + iy * t.nx];
+    return dequant(r, r.q[ix + iy * r.pitch]);
 }
 /* height of the column containing (x, y), arc_terrain.h:42-50 */
 XD float terrain_height(const TerrainK &t, const TerrainRef &r, float x, float y)
@@ -518,7 +526,49 @@ XD float terrain_height(const TerrainK &t, const TerrainRef &r, float x, float y
     float fx = floorf((x - t.ox) / t.cell), fy = floorf((y - t.oy) / t.cell);
     if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)t.nx && fy < (float)t.ny))
         return r.outside;
-    return r.h[(int)fx + (int)fy * t.nx];
+    return dequant(r, r.q[(int)fx + (int)fy * r.pitch]);
+}
+
+/* The 4 x 4 columns around a sphere's cell window, fetched ahead of use: four
+ * row loads of 4 bytes.  Covers every column collide() reads when the window
+ * spans at most 2 x 2 columns (the window plus its neighbours). */
+struct TBlock {
+    uint32_t row[4];
+    int bx, by;  /* column of row[0] byte 0 */
+    int ok;
+};
+XD void sphere_window(const TerrainK &t, V3 c, float rad, int &ix0, int &ix1, int &iy0, int &iy1)
+{
+    ix0 = (int)floorf((c.x - rad - t.ox) / t.cell);
+    ix1 = (int)floorf((c.x + rad - t.ox) / t.cell);
+    iy0 = (int)floorf((c.y - rad - t.oy) / t.cell);
+    iy1 = (int)floorf((c.y + rad - t.oy) / t.cell);
+}
+XD TBlock fetch_block(const TerrainK &t, const TerrainRef &r, V3 c, float rad)
+{
+    TBlock b;
+    b.ok = 0;
+    b.bx = b.by = 0;
+    b.row[0] = b.row[1] = b.row[2] = b.row[3] = 0u;
+    if (!(fabsf(c.x) < 1e6f && fabsf(c.y) < 1e6f))
+        return b;
+    int ix0, ix1, iy0, iy1;
+    sphere_window(t, c, rad, ix0, ix1, iy0, iy1);
+    b.bx = ix0 - 1;
+    b.by = iy0 - 1;
+    if (b.bx < -TPAD || b.by < -TPAD || b.bx + 4 > t.nx + TPAD || b.by + 4 > t.ny + TPAD)
+        return b;
+    const int8_t *p = r.q + b.bx + b.by * r.pitch;
+    for (int j = 0; j < 4; ++j)
+        __builtin_memcpy(&b.row[j], p + j * r.pitch, 4);
+    b.ok = 1;
+    return b;
+}
+XD float block_cell(const TBlock &b, const TerrainRef &r, int ix, int iy)
+{
+    int j = iy - b.by, i = ix - b.bx;
+    uint32_t row = j == 0 ? b.row[0] : j == 1 ? b.row[1] : j == 2 ? b.row[2] : b.row[3];
+    return dequant(r, (int)(int8_t)(row >> (8 * i)));
 }
 
 /* anchored static / kinetic friction, arc_phys.c:512-539.  The anchor is
@@ -562,7 +612,8 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Ancho
  * Returns nothing; updates touch / anchored / anchor state. */
 template <int KIND, int AT, class Rec, class Anchor, class PoseFn>
 XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anchored, Anchor &an,
-                const PoseFn &pose0, const TerrainK &t, const TerrainRef &tr, Rec &rec, float mus, float mud)
+                const PoseFn &pose0, const TerrainK &t, const TerrainRef &tr, const TBlock &blk, Rec &rec, float mus,
+                float mud)
 {
     float ln = 0.0f, dmax = 0.0f;
     V3 c = at_world<AT>(b, at), nmax = v3(0.0f, 0.0f, 1.0f);
@@ -571,24 +622,25 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
         anchored = 0;
         return;
     }
-    int ix0 = (int)floorf((c.x - rad - t.ox) / t.cell);
-    int ix1 = (int)floorf((c.x + rad - t.ox) / t.cell);
-    int iy0 = (int)floorf((c.y - rad - t.oy) / t.cell);
-    int iy1 = (int)floorf((c.y + rad - t.oy) / t.cell);
+    int ix0, ix1, iy0, iy1;
+    sphere_window(t, c, rad, ix0, ix1, iy0, iy1);
+    /* the prefetched block serves every read when it sits on this window */
+    const bool use_blk = blk.ok && blk.bx == ix0 - 1 && blk.by == iy0 - 1 && ix1 - ix0 <= 1 && iy1 - iy0 <= 1;
+    auto cell = [&](int ix, int iy) { return use_blk ? block_cell(blk, tr, ix, iy) : terrain_cell(t, tr, ix, iy); };
     for (int iy = iy0; iy <= iy1; ++iy) {
         for (int ix = ix0; ix <= ix1; ++ix) {
-            float top = terrain_cell(t, tr, ix, iy);
+            float top = cell(ix, iy);
             if (c.z - rad >= top)
                 continue;
             float x0 = t.ox + (float)ix * t.cell, x1 = x0 + t.cell;
             float y0 = t.oy + (float)iy * t.cell, y1 = y0 + t.cell;
             int sx = c.x > x1 ? 1 : (c.x < x0 ? -1 : 0);
             int sy = c.y > y1 ? 1 : (c.y < y0 ? -1 : 0);
-            if (sx && terrain_cell(t, tr, ix + sx, iy) >= top)
+            if (sx && cell(ix + sx, iy) >= top)
                 continue;
-            if (sy && terrain_cell(t, tr, ix, iy + sy) >= top)
+            if (sy && cell(ix, iy + sy) >= top)
                 continue;
-            if (sx && sy && terrain_cell(t, tr, ix + sx, iy + sy) >= top)
+            if (sx && sy && cell(ix + sx, iy + sy) >= top)
                 continue;
             V3 q = v3(clampf_(c.x, x0, x1), clampf_(c.y, y0, y1), minf_(c.z, top));
             V3 diff = sub(c, q);

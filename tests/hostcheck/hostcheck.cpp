@@ -19,7 +19,9 @@ struct HC {
     SimSetup su;
     std::vector<LaneWorld> w;
     std::vector<ArrayAnchors> anc;
-    std::vector<float> terrain, outside;
+    std::vector<int8_t> tq;
+    std::vector<float> tbase, outside;
+    int pitch, tcells;
     std::vector<float> cbuf;
     std::vector<int> err;
 };
@@ -34,11 +36,18 @@ HC *hc_create(int n, double dt, int substeps, int grid, double cell)
         delete h;
         return nullptr;
     }
-    const int cells = h->su.grid * h->su.grid;
+    const int g = h->su.grid;
+    h->pitch = g + 2 * TPAD;
+    h->tcells = h->pitch * h->pitch;
     h->w.resize(n);
     h->anc.assign(n, ArrayAnchors());
     memset(h->anc.data(), 0, sizeof(ArrayAnchors) * (size_t)n);
-    h->terrain.assign((size_t)n * cells, 0.0f);
+    h->tq.assign((size_t)n * h->tcells, (int8_t)Q_OUT);
+    for (int i = 0; i < n; ++i)
+        for (int iy = 0; iy < g; ++iy)
+            for (int ix = 0; ix < g; ++ix)
+                h->tq[(size_t)i * h->tcells + (ix + TPAD) + (iy + TPAD) * h->pitch] = 0;
+    h->tbase.assign(n, 0.0f);
     h->outside.assign(n, h->su.tmpl.outside);
     h->cbuf.assign((size_t)h->su.model.cslots * 6, 0.0f);
     h->err.assign(n, 0);
@@ -72,17 +81,24 @@ void hc_reset(HC *h, const int *ids, int n_ids, const unsigned *seeds, const dou
         bool gen = d > 0.0f;
         reset_draws(seeds[k], gen, n, lat.data(), hb.data(), draws);
         for (int c = 0; c < cells; ++c)
-            hb[c] = gen ? column_height(lat.data(), n, d, hb[c], c) : 0.0f;
-        TerrainRef tl;
-        tl.h = hb.data();
-        tl.outside = -100.0f;
+            hb[c] = gen ? column_step(lat.data(), n, d, hb[c], c) : 0.0f;
+        float base = 0.0f;
         if (gen) {
-            float base = terrain_height(M.terr, tl, 0.0f, 0.0f);
-            for (int c = 0; c < cells; ++c)
-                hb[c] -= base;
+            int ix = (int)floorf((0.0f - M.terr.ox) / M.terr.cell), iy = (int)floorf((0.0f - M.terr.oy) / M.terr.cell);
+            base = 0.03f * hb[ix + iy * n];
         }
-        memcpy(&h->terrain[(size_t)i * cells], hb.data(), cells * sizeof(float));
+        int8_t *q = &h->tq[(size_t)i * h->tcells];
+        for (int c = 0; c < h->tcells; ++c) {
+            int ix = c % h->pitch - TPAD, iy = c / h->pitch - TPAD;
+            q[c] = (ix >= 0 && iy >= 0 && ix < n && iy < n) ? (int8_t)(int)hb[ix + iy * n] : (int8_t)Q_OUT;
+        }
+        h->tbase[i] = base;
         h->outside[i] = -100.0f;
+        TerrainRef tl;
+        tl.q = q + TPAD + TPAD * h->pitch;
+        tl.pitch = h->pitch;
+        tl.base = base;
+        tl.outside = -100.0f;
         float ang[NJ];
         for (int j = 0; j < NJ; ++j)
             ang[j] = clampf_(R.def[j] + draws[j], R.lo[j], R.hi[j]);
@@ -116,7 +132,9 @@ void hc_step(HC *h, const double *targets)
         if (W.unstable)
             continue;
         TerrainRef tr;
-        tr.h = &h->terrain[(size_t)i * cells];
+        tr.q = &h->tq[(size_t)i * h->tcells] + TPAD + TPAD * h->pitch;
+        tr.pitch = h->pitch;
+        tr.base = h->tbase[i];
         tr.outside = h->outside[i];
         ContactStore cs;
         cs.base = h->cbuf.data();
@@ -153,10 +171,16 @@ void hc_state(const HC *h, int i, double *out)
 
 void hc_terrain(const HC *h, int i, double *out)
 {
-    const int cells = h->su.grid * h->su.grid;
-    for (int c = 0; c < cells; ++c)
-        out[c] = h->terrain[(size_t)i * cells + c];
-    out[cells] = h->outside[i];
+    const int n = h->su.grid;
+    TerrainRef r;
+    r.q = &h->tq[(size_t)i * h->tcells] + TPAD + TPAD * h->pitch;
+    r.pitch = h->pitch;
+    r.base = h->tbase[i];
+    r.outside = h->outside[i];
+    for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix)
+            out[ix + iy * n] = terrain_cell(h->su.model.terr, r, ix, iy);
+    out[n * n] = h->outside[i];
 }
 
 int hc_error(const HC *h, int i) { return h->err[i]; }
