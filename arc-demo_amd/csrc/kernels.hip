@@ -156,6 +156,67 @@ template <int BLOCK> struct LdsPoses {
     }
 };
 
+/* The columns around a world's torso, copied into LDS once per control step:
+ * TILE x TILE int8 heights of the padded grid, [column][thread].  Every lookup
+ * inside the tile reads LDS; a lookup outside it (a leg stretched beyond the
+ * tile, or a world that left the grid) reads the grid in HBM, which holds the
+ * same bytes, so the heights are identical either way. */
+constexpr int TILE = 16;
+template <int BLOCK> struct LdsTileCells {
+    const TerrainK &t;
+    const TerrainRef &r;
+    const int8_t *lt;
+    int tx, ty;
+    __device__ __forceinline__ float operator()(int ix, int iy) const
+    {
+        unsigned u = (unsigned)(ix - tx), v = (unsigned)(iy - ty);
+        if (u < (unsigned)TILE && v < (unsigned)TILE)
+            return dequant(r, lt[(u + v * TILE) * BLOCK]);
+        return terrain_cell(t, r, ix, iy);
+    }
+};
+template <int BLOCK> struct LdsTile {
+    const int8_t *lt;
+    int tx, ty;
+    __device__ __forceinline__ LdsTileCells<BLOCK> cells(const TerrainK &t, const TerrainRef &r) const
+    {
+        return LdsTileCells<BLOCK>{ t, r, lt, tx, ty };
+    }
+};
+
+/* fill this thread's tile of world i around the torso position (x, y) */
+template <int BLOCK>
+__device__ __forceinline__ LdsTile<BLOCK> fill_tile(int8_t *s_tile, const DevWorlds &d, const TerrainK &t, size_t i,
+                                                    float x, float y)
+{
+    LdsTile<BLOCK> T;
+    int8_t *lt = s_tile + threadIdx.x;
+    T.lt = lt;
+    const int n = d.tpitch - 2 * TPAD;
+    if (d.tpitch < TILE) {
+        T.tx = T.ty = 1 << 28; /* grid smaller than a tile: every lookup goes to HBM */
+        return T;
+    }
+    /* clamped in float first: a non-finite torso still yields a valid tile */
+    float fx = fminf(fmaxf((x - t.ox) / t.cell, -1.0e4f), 1.0e4f);
+    float fy = fminf(fmaxf((y - t.oy) / t.cell, -1.0e4f), 1.0e4f);
+    T.tx = min(max((int)floorf(fx) - TILE / 2, -TPAD), n + TPAD - TILE);
+    T.ty = min(max((int)floorf(fy) - TILE / 2, -TPAD), n + TPAD - TILE);
+    const int8_t *src = d.tq + i * d.tcells + (T.tx + TPAD) + (size_t)(T.ty + TPAD) * d.tpitch;
+#pragma unroll 2
+    for (int v = 0; v < TILE; ++v) {
+#pragma unroll
+        for (int w = 0; w < TILE / 4; ++w) {
+            uint32_t word;
+            __builtin_memcpy(&word, src + v * d.tpitch + 4 * w, 4);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                lt[(4 * w + b + v * TILE) * BLOCK] = (int8_t)(word >> (8 * b));
+        }
+    }
+    return T;
+}
+
 /* An index the optimiser cannot see through.  The world's SoA addresses are
  * recomputed from it for the stores at the end of the step instead of being
  * shared with the loads at its start, which would keep ~450 64-bit addresses
@@ -171,6 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
                                                 const Range *__restrict__ ranges, int n_ranges, int total)
 {
     __shared__ float s_pose[NB * 7 * BLOCK];
+    __shared__ int8_t s_tile[TILE * TILE * BLOCK];
     const Model &M = *m;
     LdsPoses<BLOCK> P0;
     P0.p = s_pose;
@@ -193,6 +255,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
         });
         if (d.unstable[i])
             continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
+        const LdsTile<BLOCK> tile = fill_tile<BLOCK>(s_tile, d, M.terr, (size_t)i, d.x[0 * S + i], d.x[1 * S + i]);
         load_world(d, i, W);
         GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
         TerrainRef tr = terrain_ref(d, i);
@@ -202,7 +265,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
         cs.base = d.cbuf;
         cs.stride = S;
         cs.world = i;
-        int ov = lane_step(W, A, P0, M, tr, cs);
+        int ov = lane_step(W, A, P0, M, tr, cs, tile);
         const size_t io = opaque_index((size_t)i);
         store_world(d, io, W);
         if (ov)

@@ -132,8 +132,14 @@ struct ContactStore {
 /* One control step (arc_world_step, csrc/arc_phys.c:707-747).
  * Returns a bit mask of bodies whose contact list overflowed its slots
  * (never expected for the robot; reported as an error by the host). */
-template <class Anchors, class Poses>
-XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const TerrainRef &tr0, const ContactStore &cs0)
+/* terrain source of a lane step: column heights read from the world's grid */
+struct GridTile {
+    XD GridCells cells(const TerrainK &t, const TerrainRef &r) const { return GridCells{ t, r }; }
+};
+
+template <class Anchors, class Poses, class Tile = GridTile>
+XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const TerrainRef &tr0, const ContactStore &cs0,
+                 const Tile &T0 = Tile())
 {
     int overflow = 0;
     /* prologue: refresh caches, clear counters (lim is precomputed per joint) */
@@ -165,16 +171,9 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
         int total = 0;
         static_for<0, NCB>([&](auto ci) { ncount[ci] = 0; });
         /* each sphere's columns are fetched while the previous sphere is solved */
-        constexpr int B0 = sphere_body(0);
-        TBlock next = fetch_block(M.terr, tr, at_world<sphere_at(0)>(W.b[B0], robot::sphere[0].at), robot::sphere[0].rad);
+        const auto cells = T0.cells(M.terr, tr);
         static_for<0, NS>([&](auto si) {
             constexpr int B = sphere_body(si);
-            const TBlock blk = next;
-            if constexpr (si + 1 < NS) {
-                constexpr int BN = sphere_body(si + 1);
-                next = fetch_block(M.terr, tr, at_world<sphere_at(si + 1)>(W.b[BN], robot::sphere[si + 1].at),
-                                   robot::sphere[si + 1].rad);
-            }
             constexpr int CI = cbody_index(B);
             int anch = (W.anchored >> si) & 1u;
             int hit = 0;
@@ -191,7 +190,7 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
             auto an = A.at(si);
             auto pose0 = [&]() { return P0.load(B); };
             collide<body_kind(B), sphere_at(si)>(W.b[B], robot::body[B], robot::sphere[si].at,
-                                                 robot::sphere[si].rad, hit, anch, an, pose0, M.terr, tr, blk, rec,
+                                                 robot::sphere[si].rad, hit, anch, an, pose0, M.terr, cells, rec,
                                                  M.mus, M.mud);
             W.anchored = (W.anchored & ~(1u << si)) | ((uint32_t)anch << si);
             W.touchp[si >> 2] += (uint32_t)hit << (8 * (si & 3));

@@ -87,10 +87,8 @@ __device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M,
         };
         AnchorRef an{ W.anc[s], W.ancn[s] };
         auto pose0 = [&]() { return W.p0[B]; };
-        TBlock none;
-        none.ok = 0;
         collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], an, pose0,
-                              M.terr, tr, none, rec, M.mus, M.mud);
+                              M.terr, GridCells{ M.terr, tr }, rec, M.mus, M.mud);
     }
     W.b[B] = b;
 }

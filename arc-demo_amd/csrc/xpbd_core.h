@@ -529,14 +529,6 @@ XD float terrain_height(const TerrainK &t, const TerrainRef &r, float x, float y
     return dequant(r, r.q[(int)fx + (int)fy * r.pitch]);
 }
 
-/* The 4 x 4 columns around a sphere's cell window, fetched ahead of use: four
- * row loads of 4 bytes.  Covers every column collide() reads when the window
- * spans at most 2 x 2 columns (the window plus its neighbours). */
-struct TBlock {
-    uint32_t row[4];
-    int bx, by;  /* column of row[0] byte 0 */
-    int ok;
-};
 XD void sphere_window(const TerrainK &t, V3 c, float rad, int &ix0, int &ix1, int &iy0, int &iy1)
 {
     ix0 = (int)floorf((c.x - rad - t.ox) / t.cell);
@@ -544,32 +536,13 @@ XD void sphere_window(const TerrainK &t, V3 c, float rad, int &ix0, int &ix1, in
     iy0 = (int)floorf((c.y - rad - t.oy) / t.cell);
     iy1 = (int)floorf((c.y + rad - t.oy) / t.cell);
 }
-XD TBlock fetch_block(const TerrainK &t, const TerrainRef &r, V3 c, float rad)
-{
-    TBlock b;
-    b.ok = 0;
-    b.bx = b.by = 0;
-    b.row[0] = b.row[1] = b.row[2] = b.row[3] = 0u;
-    if (!(fabsf(c.x) < 1e6f && fabsf(c.y) < 1e6f))
-        return b;
-    int ix0, ix1, iy0, iy1;
-    sphere_window(t, c, rad, ix0, ix1, iy0, iy1);
-    b.bx = ix0 - 1;
-    b.by = iy0 - 1;
-    if (b.bx < -TPAD || b.by < -TPAD || b.bx + 4 > t.nx + TPAD || b.by + 4 > t.ny + TPAD)
-        return b;
-    const int8_t *p = r.q + b.bx + b.by * r.pitch;
-    for (int j = 0; j < 4; ++j)
-        __builtin_memcpy(&b.row[j], p + j * r.pitch, 4);
-    b.ok = 1;
-    return b;
-}
-XD float block_cell(const TBlock &b, const TerrainRef &r, int ix, int iy)
-{
-    int j = iy - b.by, i = ix - b.bx;
-    uint32_t row = j == 0 ? b.row[0] : j == 1 ? b.row[1] : j == 2 ? b.row[2] : b.row[3];
-    return dequant(r, (int)(int8_t)(row >> (8 * i)));
-}
+
+/* column heights straight from the world's grid */
+struct GridCells {
+    const TerrainK &t;
+    const TerrainRef &r;
+    XD float operator()(int ix, int iy) const { return terrain_cell(t, r, ix, iy); }
+};
 
 /* anchored static / kinetic friction, arc_phys.c:512-539.  The anchor is
  * read and written through `an` (only when the sphere touches), the
@@ -609,11 +582,11 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Ancho
 
 /* One sphere against the column grid, each contact solved at once and
  * recorded through rec(local, n) for the velocity pass; arc_phys.c:542-610.
- * Returns nothing; updates touch / anchored / anchor state. */
-template <int KIND, int AT, class Rec, class Anchor, class PoseFn>
+ * cell(ix, iy) gives a column's height (any source holding the same int8
+ * grid).  Updates touch / anchored / anchor state. */
+template <int KIND, int AT, class Rec, class Anchor, class PoseFn, class Cells>
 XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anchored, Anchor &an,
-                const PoseFn &pose0, const TerrainK &t, const TerrainRef &tr, const TBlock &blk, Rec &rec, float mus,
-                float mud)
+                const PoseFn &pose0, const TerrainK &t, const Cells &cell, Rec &rec, float mus, float mud)
 {
     float ln = 0.0f, dmax = 0.0f;
     V3 c = at_world<AT>(b, at), nmax = v3(0.0f, 0.0f, 1.0f);
@@ -624,9 +597,6 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
     }
     int ix0, ix1, iy0, iy1;
     sphere_window(t, c, rad, ix0, ix1, iy0, iy1);
-    /* the prefetched block serves every read when it sits on this window */
-    const bool use_blk = blk.ok && blk.bx == ix0 - 1 && blk.by == iy0 - 1 && ix1 - ix0 <= 1 && iy1 - iy0 <= 1;
-    auto cell = [&](int ix, int iy) { return use_blk ? block_cell(blk, tr, ix, iy) : terrain_cell(t, tr, ix, iy); };
     for (int iy = iy0; iy <= iy1; ++iy) {
         for (int ix = ix0; ix <= ix1; ++ix) {
             float top = cell(ix, iy);

@@ -173,3 +173,26 @@ def test_wave_form_matches_lane_form(native, difficulty):
         assert same(a, b).all(), f"wave vs lane after step {k + 1}"
     c = _states(o, n)
     assert np.all(np.abs(a - c) / (np.abs(c) + 1.0) <= 1e-4)
+
+
+@pytest.mark.parametrize("grid,cell", [(8, 0.4), (16, 0.4), (20, 0.25)])
+def test_small_grids(native, grid, cell):
+    """Grids where the robot's legs reach the border and beyond: on the 8-column
+    grid no LDS tile fits and every column is read from HBM; on the others the
+    tile is clamped to the padded grid and the legs read columns outside it."""
+    n = 64
+    diff = P.difficulties(1, n)
+    g = native.Sim(n, grid=grid, cell=cell)
+    o = P.Lib("o32").sim(n, grid=grid, cell=cell)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), diff)
+    o.reset(ids, ids.astype(np.uint32), diff)
+    for i in range(n):
+        assert np.array_equal(g.terrain(i), o.terrain(i)), f"terrain of world {i}"
+    t = action_stream(n, 1)[0]
+    g.step(t)
+    o.step(t)
+    a, b = _states(g, n), _states(o, n)
+    err = np.abs(a - b) / (np.abs(b) + 1.0)
+    print(f"grid {grid} cell {cell}: max rel err {float(err.max()):.3g}, bit-exact worlds {same(a, b).all(1).mean():.3f}")
+    assert float(err.max()) <= REL
