@@ -1,0 +1,347 @@
+/*
+ * batch.cpp -- the batched form of libarcsim: one launch steps the worlds of
+ * several requests (uneven world ranges), optionally as a persistent grid
+ * that takes work from a device counter, optionally replayed from a captured
+ * hipGraph; and the autotuner that picks the launch configuration (workgroup
+ * size, LDS layout, work distribution) by timing this handle's own worlds.
+ *
+ * The request table lives in device memory (StepBatch) and is written before
+ * each launch, so one captured graph serves every batch: the graph holds the
+ * kernels, the table holds what they step.
+ */
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+#include "sim_internal.h"
+
+namespace {
+
+constexpr int RING = 4; /* pinned request tables in flight */
+
+struct LaunchKey {
+    const float *targets;
+    int task, block, tile, grid, persistent;
+    bool operator==(const LaunchKey &o) const
+    {
+        return targets == o.targets && task == o.task && block == o.block && tile == o.tile && grid == o.grid
+               && persistent == o.persistent;
+    }
+};
+
+} /* namespace */
+
+struct BatchState {
+    StepBatch *d_batch = nullptr;
+    StepBatch *h_batch[RING] = {};
+    hipEvent_t ev[RING] = {};
+    int slot = 0;
+    int *d_work = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    LaunchKey key{};
+    int captures = 0;
+};
+
+void batch_free(arc_sim *s)
+{
+    BatchState *b = s->batch;
+    if (!b)
+        return;
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    for (int k = 0; k < RING; ++k) {
+        if (b->ev[k])
+            (void)hipEventDestroy(b->ev[k]);
+        if (b->h_batch[k])
+            (void)hipHostFree(b->h_batch[k]);
+    }
+    if (b->d_batch)
+        (void)hipFree(b->d_batch);
+    if (b->d_work)
+        (void)hipFree(b->d_work);
+    delete b;
+    s->batch = nullptr;
+}
+
+static BatchState *batch_of(arc_sim *s)
+{
+    if (s->batch)
+        return s->batch;
+    BatchState *b = new (std::nothrow) BatchState();
+    if (!b)
+        return nullptr;
+    s->batch = b;
+    hipError_t e = hipMalloc((void **)&b->d_batch, sizeof(StepBatch));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&b->d_work, 64);
+    for (int k = 0; e == hipSuccess && k < RING; ++k) {
+        e = hipHostMalloc((void **)&b->h_batch[k], sizeof(StepBatch), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        set_error(s, (int)e, "batch state", e);
+        batch_free(s);
+        return nullptr;
+    }
+    return b;
+}
+
+/* grid of a lane-form launch over `total` items (graph: the worst case N) */
+static int grid_for(const arc_sim *s, int total)
+{
+    if (s->persistent) {
+        int r = step_resident_blocks(s->block, s->tile);
+        return r > 0 ? r : (total + s->block - 1) / s->block;
+    }
+    if (s->grid_blocks > 0)
+        return s->grid_blocks;
+    return (total + s->block - 1) / s->block;
+}
+
+/* the kernels of one batched control step, on the handle's stream */
+static hipError_t enqueue(arc_sim *s, BatchState *b, const float *targets, int task, int grid)
+{
+    hipError_t e;
+    if (s->persistent && (e = launch_fill_int(b->d_work, 0, 1, s->stream)) != hipSuccess)
+        return e;
+    if ((e = launch_step(s->dev, s->d_model, targets, b->d_batch, 0, s->block, s->tile, grid,
+                         s->persistent ? b->d_work : nullptr, s->stream)) != hipSuccess)
+        return e;
+    if (!task)
+        return hipSuccess;
+    if ((e = launch_fill_int(s->ts.count, 0, 1, s->stream)) != hipSuccess
+        || (e = launch_falls(s->dev, s->d_model, s->task, s->ts, b->d_batch, s->stream)) != hipSuccess)
+        return e;
+    return launch_reset(s->dev, s->d_model, s->d_reset, s->ts.list, s->ts.seeds, nullptr, s->ts.diff, 0, s->ts.count,
+                        s->reset_blocks, s->stream);
+}
+
+static hipError_t capture(arc_sim *s, BatchState *b, const LaunchKey &k)
+{
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return e;
+    hipError_t e1 = enqueue(s, b, k.targets, k.task, k.grid);
+    hipError_t e2 = hipStreamEndCapture(s->stream, &b->graph);
+    if (e1 != hipSuccess)
+        return e1;
+    if (e2 != hipSuccess)
+        return e2;
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return e;
+    b->key = k;
+    b->captures++;
+    return hipSuccess;
+}
+
+extern "C" {
+
+int arcx_set_layout(arc_sim *s, int tile)
+{
+    if (tile != 0 && tile != 16)
+        return -1;
+    s->tile = tile;
+    return 0;
+}
+
+int arcx_set_persistent(arc_sim *s, int on)
+{
+    s->persistent = on ? 1 : 0;
+    return 0;
+}
+
+int arcx_get_launch(const arc_sim *s, int *out4)
+{
+    out4[0] = s->block;
+    out4[1] = s->tile;
+    out4[2] = s->persistent ? step_resident_blocks(s->block, s->tile) : s->grid_blocks;
+    out4[3] = s->persistent;
+    return 0;
+}
+
+int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
+                    int graph)
+{
+    if (n_req < 0 || n_req > MAX_BATCH || (n_req && (!starts || !counts)) || !targets_dev) {
+        set_error(s, -1, "arcx_step_batch: request list", hipSuccess);
+        return -1;
+    }
+    /* ranges inside [0, N) and disjoint: overlapping ranges would race */
+    std::vector<std::pair<int, int>> iv;
+    iv.reserve(n_req);
+    for (int r = 0; r < n_req; ++r) {
+        if (counts[r] < 0 || starts[r] < 0 || starts[r] > s->n - counts[r]) {
+            set_error(s, -1, "arcx_step_batch: range outside the sim", hipSuccess);
+            return -1;
+        }
+        if (counts[r])
+            iv.emplace_back(starts[r], counts[r]);
+    }
+    std::sort(iv.begin(), iv.end());
+    for (size_t k = 1; k < iv.size(); ++k)
+        if (iv[k - 1].first + iv[k - 1].second > iv[k].first) {
+            set_error(s, -1, "arcx_step_batch: overlapping ranges", hipSuccess);
+            return -1;
+        }
+    if (s->form != 0) {
+        set_error(s, -1, "arcx_step_batch: the batched form is the lane form", hipSuccess);
+        return -1;
+    }
+    BatchState *b = batch_of(s);
+    if (!b)
+        return -1;
+    const int slot = b->slot;
+    b->slot = (slot + 1) % RING;
+    HIPCHK_RET(s, hipEventSynchronize(b->ev[slot]), "arcx_step_batch", -1);
+    StepBatch *h = b->h_batch[slot];
+    int total = 0, m = 0;
+    for (int r = 0; r < n_req; ++r) {
+        if (!counts[r])
+            continue;
+        h->r[m].start = starts[r];
+        h->r[m].count = counts[r];
+        h->r[m].prefix = total;
+        h->r[m].pad = 0;
+        total += counts[r];
+        ++m;
+    }
+    h->n_ranges = m;
+    h->total = total;
+    if (m == 0) {
+        /* an empty table still steps nothing: the kernels read total = 0 */
+        h->r[0].start = h->r[0].count = h->r[0].prefix = 0;
+    }
+    const size_t bytes = offsetof(StepBatch, r) + sizeof(Range) * (size_t)std::max(m, 1);
+    HIPCHK_RET(s, hipMemcpyAsync(b->d_batch, h, bytes, hipMemcpyHostToDevice, s->stream), "arcx_step_batch table", -1);
+    HIPCHK_RET(s, hipEventRecord(b->ev[slot], s->stream), "arcx_step_batch", -1);
+    if (!graph) {
+        if (total == 0 && !task)
+            return 0;
+        HIPCHK_RET(s, enqueue(s, b, targets_dev, task, grid_for(s, std::max(total, 1))), "arcx_step_batch launch", -1);
+        return 0;
+    }
+    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, s->tile, grid_for(s, std::max(s->n, 1)), s->persistent };
+    if (!b->exec || !(k == b->key))
+        HIPCHK_RET(s, capture(s, b, k), "arcx_step_batch capture", -1);
+    HIPCHK_RET(s, hipGraphLaunch(b->exec, s->stream), "arcx_step_batch graph", -1);
+    return 0;
+}
+
+int arcx_graph_captures(const arc_sim *s) { return s->batch ? s->batch->captures : 0; }
+
+/* Times every launch configuration of the lane form on this handle's worlds
+ * and keeps the fastest.  The device state is copied aside first and put
+ * back afterwards, so the worlds end where they started. */
+int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
+{
+    if (s->n <= 0 || steps <= 0) {
+        set_error(s, -1, "arcx_autotune: nothing to time", hipSuccess);
+        return -1;
+    }
+    if (s->form != 0) {
+        set_error(s, -1, "arcx_autotune: the tuner configures the lane form", hipSuccess);
+        return -1;
+    }
+    const size_t nt = (size_t)s->n * NJ;
+    std::vector<float> ht(nt);
+    {
+        /* default pose plus 0.6 x uniform actions from the action stream (seed 0) */
+        std::vector<unsigned> u(nt);
+        arcx_rng_stream(0u, (long long)nt, u.data());
+        for (size_t k = 0; k < nt; ++k) {
+            double a = (double)(u[k] >> 8) * (1.0 / 8388608.0) - 1.0;
+            ht[k] = (float)((double)s->su.info.def[k % NJ] + 0.6 * a);
+        }
+    }
+    float *dt = nullptr;
+    void *snap = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = -1;
+    const int keep_block = s->block, keep_tile = s->tile, keep_grid = s->grid_blocks, keep_pers = s->persistent;
+    int best[4] = { keep_block, keep_tile, keep_grid, keep_pers };
+    double best_ms = 1e30;
+    do {
+        if (hipMalloc((void **)&dt, nt * sizeof(float)) != hipSuccess || hipMalloc(&snap, s->dev.bytes) != hipSuccess
+            || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            set_error(s, -1, "arcx_autotune: allocation", hipSuccess);
+            break;
+        }
+        bool ok = hipMemcpyAsync(dt, ht.data(), nt * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
+                  && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
+                  && hipStreamSynchronize(s->stream) == hipSuccess;
+        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 };
+        for (int bi = 0; bi < 3 && ok; ++bi)
+            for (int ti = 0; ti < 2 && ok; ++ti)
+                for (int pi = 0; pi < 2 && ok; ++pi) {
+                    s->block = blocks[bi];
+                    s->tile = tiles[ti];
+                    s->grid_blocks = 0;
+                    s->persistent = pi;
+                    const int start = 0, count = s->n;
+                    /* from the saved state: one untimed step, then `steps` timed ones */
+                    float ms = 0.0f;
+                    ok = hipMemcpyAsync(s->dev.base, snap, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream)
+                         == hipSuccess;
+                    ok = ok && arcx_step_batch(s, &start, &count, 1, dt, 0, 0) == 0;
+                    ok = ok && hipEventRecord(e0, s->stream) == hipSuccess;
+                    for (int k = 0; k < steps && ok; ++k)
+                        ok = arcx_step_batch(s, &start, &count, 1, dt, 0, 0) == 0;
+                    ok = ok && hipEventRecord(e1, s->stream) == hipSuccess;
+                    ok = ok && hipEventSynchronize(e1) == hipSuccess;
+                    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                    ms /= (float)steps;
+                    if (ok && ms < best_ms) {
+                        best_ms = ms;
+                        best[0] = s->block;
+                        best[1] = s->tile;
+                        best[2] = 0;
+                        best[3] = s->persistent;
+                    }
+                }
+        if (!ok) {
+            set_error(s, -1, "arcx_autotune: timing run", hipSuccess);
+            break;
+        }
+        rc = 0;
+    } while (0);
+    if (snap)
+        (void)hipMemcpyAsync(s->dev.base, snap, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipStreamSynchronize(s->stream);
+    if (rc == 0) {
+        s->block = best[0];
+        s->tile = best[1];
+        s->grid_blocks = best[2];
+        s->persistent = best[3];
+        arcx_get_launch(s, out4);
+        if (ms_out)
+            *ms_out = best_ms;
+    } else {
+        s->block = keep_block;
+        s->tile = keep_tile;
+        s->grid_blocks = keep_grid;
+        s->persistent = keep_pers;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dt) (void)hipFree(dt);
+    if (snap) (void)hipFree(snap);
+    return rc;
+}
+
+} /* extern "C" */

@@ -27,6 +27,8 @@ struct DevWorlds {
     int cells;        /* nx * ny terrain columns per world */
     int tpitch;       /* nx + 2 * TPAD: row pitch of the stored terrain */
     int tcells;       /* tpitch^2 bytes of stored terrain per world */
+    void *base;       /* the one allocation holding every array below */
+    size_t bytes;     /* its size */
     float *x;         /* [NB*3][stride] body position */
     float *q;         /* [NB*4][stride] body orientation w,x,y,z */
     float *v;         /* [NB*3][stride] linear velocity */
@@ -82,11 +84,31 @@ struct TaskState {
     unsigned long long *stats; /* [5] terminations per cause (index 0: steps) */
 };
 
-/* Slices of worlds a persistent launch steps: worlds [start, start+count)
+/* Slices of worlds a batched launch steps: worlds [start, start+count)
  * for each range; prefix = sum of the counts before it. */
 struct Range {
     int start, count, prefix, pad;
 };
+/* requests per batched launch */
+constexpr int MAX_BATCH = 256;
+/* the batch a launch reads from device memory when it starts */
+struct StepBatch {
+    int n_ranges, total, pad[2];
+    Range r[MAX_BATCH];
+};
+/* world index of work item t of a batch: binary search over the prefixes */
+__device__ __forceinline__ int batch_world(const StepBatch *__restrict__ b, int n_ranges, int t)
+{
+    int lo = 0, hi = n_ranges - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (b->r[mid].prefix <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return b->r[lo].start + (t - b->r[lo].prefix);
+}
 
 /* Kernel launchers (kernels.hip).  All are asynchronous on `stream`. */
 size_t dev_bytes_per_world(int grid, int cslots);
@@ -104,8 +126,22 @@ __host__ __device__ inline TerrainRef terrain_ref(const DevWorlds &d, size_t i)
 void dev_free(DevWorlds &d);
 hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, float outside, hipStream_t s);
 /* targets: n*NJ world-major, double (host ABI) or float (device inputs) */
-hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *targets_d, const float *targets_f,
-                       const Range *ranges, int n_ranges, int total, int block, int grid_blocks, hipStream_t s);
+/* step kernel, lane form.  batch (device) or worlds 0..total-1; tile 0 or 16
+ * (LDS terrain tile side); grid_blocks 0 = one work item per thread; work
+ * (device counter, zeroed beforehand) selects the persistent distribution. */
+hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch, int total,
+                       int block, int tile, int grid_blocks, int *work, hipStream_t s);
+hipError_t launch_step_b64(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
+                           int total, int tile, int grid_blocks, int *work, hipStream_t s);
+hipError_t launch_step_b128(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
+                            int total, int tile, int grid_blocks, int *work, hipStream_t s);
+hipError_t launch_step_b256(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
+                            int total, int tile, int grid_blocks, int *work, hipStream_t s);
+/* workgroups of the step kernel (block, tile) resident on the device at once */
+int step_resident_blocks(int block, int tile);
+int step_resident_blocks_b64(int tile);
+int step_resident_blocks_b128(int tile);
+int step_resident_blocks_b256(int tile);
 /* Resets ids[k] with seeds[k] and difficulty (double or float array), for k <
  * *count_dev when count_dev is given (entries decided on the device), else
  * k < n_ids; at most `max_blocks` blocks loop over the entries. */
@@ -116,7 +152,9 @@ hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_
 hipError_t launch_fill_int(int *p, int v, int n, hipStream_t s);
 /* Fall rules after a step; fallen worlds are appended to the task list
  * (ts.count must be zero before). */
-hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts, hipStream_t s);
+/* fall rules over the worlds of batch (device), or all worlds when null */
+hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts,
+                        const StepBatch *batch, hipStream_t s);
 hipError_t launch_observe(const DevWorlds &d, const Model *m_dev, double *root, double *rot, double *joints,
                           double *contacts, int *flags, hipStream_t s);
 hipError_t launch_heights(const DevWorlds &d, const Model *m_dev, const double *offsets, int k, double *out,

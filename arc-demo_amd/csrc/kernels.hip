@@ -1,9 +1,7 @@
 /*
  * kernels.hip -- HIP kernels of the batched engine for gfx950.
  *
- *   k_step          one control step per world, one world per lane
- *                   (lane_engine.h); a grid-stride loop over a list of world
- *                   ranges, so one launch serves uneven per-request batches
+ *   (k_step, the lane-form step kernel, lives in step_kernel.h)
  *   k_reset         regenerates terrain and re-poses listed worlds, one
  *                   wavefront per world (csrc/arc_sim.c:88-121)
  *   k_observe / k_heights / k_terrain_at / k_dump   read-outs
@@ -41,6 +39,8 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e != hipSuccess)
         return e;
+    d.base = p;
+    d.bytes = bytes;
     float *f = (float *)p;
     d.x = f; f += NB * 3 * S;
     d.q = f; f += NB * 4 * S;
@@ -65,241 +65,36 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
 
 void dev_free(DevWorlds &d)
 {
-    if (d.x)
-        (void)hipFree(d.x);
+    if (d.base)
+        (void)hipFree(d.base);
+    d.base = nullptr;
     d.x = nullptr;
 }
 
 /* ------------------------------------------------------------------ */
 
-__device__ __forceinline__ void load_world(const DevWorlds &d, size_t i, LaneWorld &W)
+
+hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch, int total,
+                       int block, int tile, int grid_blocks, int *work, hipStream_t s)
 {
-    const size_t S = d.stride;
-    static_for<0, NB>([&](auto b) {
-        W.b[b].x = v3(d.x[(b * 3 + 0) * S + i], d.x[(b * 3 + 1) * S + i], d.x[(b * 3 + 2) * S + i]);
-        W.b[b].q = q4(d.q[(b * 4 + 0) * S + i], d.q[(b * 4 + 1) * S + i], d.q[(b * 4 + 2) * S + i],
-                      d.q[(b * 4 + 3) * S + i]);
-        W.b[b].v = v3(d.v[(b * 3 + 0) * S + i], d.v[(b * 3 + 1) * S + i], d.v[(b * 3 + 2) * S + i]);
-        W.b[b].w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
-    });
-    W.anchored = d.anchored[i];
-    W.unstable = d.unstable[i];
-    W.time = d.time[i];
-}
-
-__device__ __forceinline__ void store_world(const DevWorlds &d, size_t i, const LaneWorld &W)
-{
-    const size_t S = d.stride;
-    static_for<0, NB>([&](auto b) {
-        d.x[(b * 3 + 0) * S + i] = W.b[b].x.x; d.x[(b * 3 + 1) * S + i] = W.b[b].x.y; d.x[(b * 3 + 2) * S + i] = W.b[b].x.z;
-        d.q[(b * 4 + 0) * S + i] = W.b[b].q.w; d.q[(b * 4 + 1) * S + i] = W.b[b].q.x;
-        d.q[(b * 4 + 2) * S + i] = W.b[b].q.y; d.q[(b * 4 + 3) * S + i] = W.b[b].q.z;
-        d.v[(b * 3 + 0) * S + i] = W.b[b].v.x; d.v[(b * 3 + 1) * S + i] = W.b[b].v.y; d.v[(b * 3 + 2) * S + i] = W.b[b].v.z;
-        d.w[(b * 3 + 0) * S + i] = W.b[b].w.x; d.w[(b * 3 + 1) * S + i] = W.b[b].w.y; d.w[(b * 3 + 2) * S + i] = W.b[b].w.z;
-    });
-    static_for<0, NJ>([&](auto j) {
-        d.jout[(0 * NJ + j) * S + i] = W.angle[j];
-        d.jout[(1 * NJ + j) * S + i] = W.rate[j];
-        d.jout[(2 * NJ + j) * S + i] = W.torque[j];
-    });
-    d.anchored[i] = W.anchored;
-    static_for<0, NS>([&](auto s) { d.touch[s * S + i] = (uint8_t)touch_of(W, s); });
-    d.unstable[i] = W.unstable;
-    d.time[i] = W.time;
-}
-
-/* friction anchor of sphere s of world i, read and written in its HBM
- * arrays only while the sphere is in contact */
-struct GlobalAnchor {
-    float *a, *n;
-    size_t S, i;
-    int s;
-    __device__ __forceinline__ V3 get(const float *p) const
-    {
-        return v3(p[(s * 3 + 0) * S + i], p[(s * 3 + 1) * S + i], p[(s * 3 + 2) * S + i]);
-    }
-    __device__ __forceinline__ void put(float *p, V3 v)
-    {
-        p[(s * 3 + 0) * S + i] = v.x;
-        p[(s * 3 + 1) * S + i] = v.y;
-        p[(s * 3 + 2) * S + i] = v.z;
-    }
-    __device__ __forceinline__ V3 point() const { return get(a); }
-    __device__ __forceinline__ V3 normal() const { return get(n); }
-    __device__ __forceinline__ void set_point(V3 v) { put(a, v); }
-    __device__ __forceinline__ void set_normal(V3 v) { put(n, v); }
-};
-struct GlobalAnchors {
-    float *a, *n;
-    size_t S, i;
-    __device__ __forceinline__ GlobalAnchor at(int s) const { return GlobalAnchor{ a, n, S, i, s }; }
-    __device__ __forceinline__ GlobalAnchors fresh() const { return GlobalAnchors{ a, n, S, launder(i) }; }
-};
-
-/* substep-start poses of the block's worlds in LDS: [body * 7 + field][thread] */
-template <int BLOCK> struct LdsPoses {
-    float *p;
-    int tid;
-    __device__ __forceinline__ void store(int b, const Pose0 &v)
-    {
-        float *q = p + b * 7 * BLOCK + tid;
-        q[0] = v.x0.x; q[BLOCK] = v.x0.y; q[2 * BLOCK] = v.x0.z;
-        q[3 * BLOCK] = v.q0.w; q[4 * BLOCK] = v.q0.x; q[5 * BLOCK] = v.q0.y; q[6 * BLOCK] = v.q0.z;
-    }
-    __device__ __forceinline__ Pose0 load(int b) const
-    {
-        const float *q = p + b * 7 * BLOCK + tid;
-        Pose0 v;
-        v.x0 = v3(q[0], q[BLOCK], q[2 * BLOCK]);
-        v.q0 = q4(q[3 * BLOCK], q[4 * BLOCK], q[5 * BLOCK], q[6 * BLOCK]);
-        return v;
-    }
-};
-
-/* The columns around a world's torso, copied into LDS once per control step:
- * TILE x TILE int8 heights of the padded grid, [column][thread].  Every lookup
- * inside the tile reads LDS; a lookup outside it (a leg stretched beyond the
- * tile, or a world that left the grid) reads the grid in HBM, which holds the
- * same bytes, so the heights are identical either way. */
-constexpr int TILE = 16;
-template <int BLOCK> struct LdsTileCells {
-    const TerrainK &t;
-    const TerrainRef &r;
-    const int8_t *lt;
-    int tx, ty;
-    __device__ __forceinline__ float operator()(int ix, int iy) const
-    {
-        unsigned u = (unsigned)(ix - tx), v = (unsigned)(iy - ty);
-        if (u < (unsigned)TILE && v < (unsigned)TILE)
-            return dequant(r, lt[(u + v * TILE) * BLOCK]);
-        return terrain_cell(t, r, ix, iy);
-    }
-};
-template <int BLOCK> struct LdsTile {
-    const int8_t *lt;
-    int tx, ty;
-    __device__ __forceinline__ LdsTileCells<BLOCK> cells(const TerrainK &t, const TerrainRef &r) const
-    {
-        return LdsTileCells<BLOCK>{ t, r, lt, tx, ty };
-    }
-};
-
-/* fill this thread's tile of world i around the torso position (x, y) */
-template <int BLOCK>
-__device__ __forceinline__ LdsTile<BLOCK> fill_tile(int8_t *s_tile, const DevWorlds &d, const TerrainK &t, size_t i,
-                                                    float x, float y)
-{
-    LdsTile<BLOCK> T;
-    int8_t *lt = s_tile + threadIdx.x;
-    T.lt = lt;
-    const int n = d.tpitch - 2 * TPAD;
-    if (d.tpitch < TILE) {
-        T.tx = T.ty = 1 << 28; /* grid smaller than a tile: every lookup goes to HBM */
-        return T;
-    }
-    /* clamped in float first: a non-finite torso still yields a valid tile */
-    float fx = fminf(fmaxf((x - t.ox) / t.cell, -1.0e4f), 1.0e4f);
-    float fy = fminf(fmaxf((y - t.oy) / t.cell, -1.0e4f), 1.0e4f);
-    T.tx = min(max((int)floorf(fx) - TILE / 2, -TPAD), n + TPAD - TILE);
-    T.ty = min(max((int)floorf(fy) - TILE / 2, -TPAD), n + TPAD - TILE);
-    const int8_t *src = d.tq + i * d.tcells + (T.tx + TPAD) + (size_t)(T.ty + TPAD) * d.tpitch;
-#pragma unroll 2
-    for (int v = 0; v < TILE; ++v) {
-#pragma unroll
-        for (int w = 0; w < TILE / 4; ++w) {
-            uint32_t word;
-            __builtin_memcpy(&word, src + v * d.tpitch + 4 * w, 4);
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                lt[(4 * w + b + v * TILE) * BLOCK] = (int8_t)(word >> (8 * b));
-        }
-    }
-    return T;
-}
-
-/* An index the optimiser cannot see through.  The world's SoA addresses are
- * recomputed from it for the stores at the end of the step instead of being
- * shared with the loads at its start, which would keep ~450 64-bit addresses
- * live across the whole step. */
-__device__ __forceinline__ size_t opaque_index(size_t i)
-{
-    asm volatile("" : "+v"(i));
-    return i;
-}
-
-template <class TT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__restrict__ m, const TT *__restrict__ targets,
-                                                const Range *__restrict__ ranges, int n_ranges, int total)
-{
-    __shared__ float s_pose[NB * 7 * BLOCK];
-    __shared__ int8_t s_tile[TILE * TILE * BLOCK];
-    const Model &M = *m;
-    LdsPoses<BLOCK> P0;
-    P0.p = s_pose;
-    P0.tid = threadIdx.x;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        int i = t;
-        if (ranges) {
-            int r = 0;
-            while (r + 1 < n_ranges && ranges[r + 1].prefix <= t)
-                ++r;
-            i = ranges[r].start + (t - ranges[r].prefix);
-        }
-        const size_t S = d.stride;
-        LaneWorld W;
-        /* targets clamped to the limits, csrc/arc_sim.c:132-134 */
-        static_for<0, NJ>([&](auto j) {
-            float tg = (float)targets[(size_t)i * NJ + j];
-            W.target[j] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
-            d.target[j * S + i] = W.target[j];
-        });
-        if (d.unstable[i])
-            continue; /* frozen until reset, csrc/arc_sim.c:135-136 */
-        const LdsTile<BLOCK> tile = fill_tile<BLOCK>(s_tile, d, M.terr, (size_t)i, d.x[0 * S + i], d.x[1 * S + i]);
-        load_world(d, i, W);
-        GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
-        TerrainRef tr = terrain_ref(d, i);
-        tr.base = d.tbase[i];
-        tr.outside = d.outside[i];
-        ContactStore cs;
-        cs.base = d.cbuf;
-        cs.stride = S;
-        cs.world = i;
-        int ov = lane_step(W, A, P0, M, tr, cs, tile);
-        const size_t io = opaque_index((size_t)i);
-        store_world(d, io, W);
-        if (ov)
-            d.err[io] |= ov;
-    }
-}
-
-hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const double *targets_d, const float *targets_f,
-                       const Range *ranges, int n_ranges, int total, int block, int grid_blocks, hipStream_t s)
-{
-    if (total <= 0)
+    if (!batch && total <= 0)
         return hipSuccess;
-    if (block <= 0)
-        block = 64;
-    if (grid_blocks <= 0)
-        grid_blocks = (total + block - 1) / block;
-#define ARC_STEP_LAUNCH(B)                                                                                     \
-    case B:                                                                                                    \
-        if (targets_d)                                                                                         \
-            hipLaunchKernelGGL((k_step<double, B>), dim3(grid_blocks), dim3(B), 0, s, d, m_dev, targets_d, ranges,   \
-                               n_ranges, total);                                                               \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_step<float, B>), dim3(grid_blocks), dim3(B), 0, s, d, m_dev, targets_f, ranges,    \
-                               n_ranges, total);                                                               \
-        break;
     switch (block) {
-        ARC_STEP_LAUNCH(64)
-        ARC_STEP_LAUNCH(128)
-        ARC_STEP_LAUNCH(256)
-    default:
-        return hipErrorInvalidValue;
+    case 64: return launch_step_b64(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+    case 128: return launch_step_b128(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+    case 256: return launch_step_b256(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+    default: return hipErrorInvalidValue;
     }
-#undef ARC_STEP_LAUNCH
-    return hipGetLastError();
+}
+
+int step_resident_blocks(int block, int tile)
+{
+    switch (block) {
+    case 64: return step_resident_blocks_b64(tile);
+    case 128: return step_resident_blocks_b128(tile);
+    case 256: return step_resident_blocks_b256(tile);
+    default: return 0;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -410,12 +205,14 @@ hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_
 
 /* fall rules of the task after a step, arc/task.py:180-203; comparisons in
  * binary64 on the binary32 state, as the task makes them on float64 copies */
-__global__ void k_falls(DevWorlds d, const Model *__restrict__ m, TaskK tk, TaskState ts)
+__global__ void k_falls(DevWorlds d, const Model *__restrict__ m, TaskK tk, TaskState ts,
+                        const StepBatch *__restrict__ batch)
 {
     const size_t S = d.stride;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (batch ? batch->total : d.n))
         return;
+    const int i = batch ? batch_world(batch, batch->n_ranges, t) : t;
     float r[13];
     for (int c = 0; c < 3; ++c) r[c] = d.x[c * S + i];
     for (int c = 0; c < 4; ++c) r[3 + c] = d.q[c * S + i];
@@ -465,11 +262,12 @@ hipError_t launch_fill_int(int *p, int v, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts, hipStream_t s)
+hipError_t launch_falls(const DevWorlds &d, const Model *m_dev, const TaskK &tk, const TaskState &ts,
+                        const StepBatch *batch, hipStream_t s)
 {
     if (d.n <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_falls, dim3((d.n + 255) / 256), dim3(256), 0, s, d, m_dev, tk, ts);
+    hipLaunchKernelGGL(k_falls, dim3((d.n + 255) / 256), dim3(256), 0, s, d, m_dev, tk, ts, batch);
     return hipGetLastError();
 }
 

@@ -12,48 +12,11 @@
 #include <algorithm>
 #include <vector>
 
-#include "../../include/arc_gpu.h"
-#include "device_state.h"
 #include "model.h"
-#include "setup.h"
-#include "wave_tables.h"
 #include "reset_engine.h"
+#include "sim_internal.h"
 
-using namespace xpbd;
-
-struct arc_sim {
-    int n;
-    int substeps;
-    double dt_d, cell_d;
-    SimSetup su;
-    DevWorlds dev;
-    Model *d_model;
-    ResetK *d_reset;
-    TemplateK *d_tmpl;
-    hipStream_t stream;
-    int device;
-    /* staging */
-    double *d_io;          /* device: max(N*12, N*64, N*k) doubles */
-    size_t d_io_doubles;
-    char *d_aux;           /* device: ids/seeds/difficulty/offsets */
-    size_t d_aux_bytes;
-    char *h_pin;           /* pinned host staging */
-    size_t h_pin_bytes;
-    float *d_dump;
-    arc_world *mirror;
-    int block, grid_blocks;
-    int form;              /* 0: one world per lane, 1: one world per wavefront */
-    WaveK wave;
-    WaveK *d_wave;
-    TaskK task;
-    TaskState ts;
-    void *ts_mem;
-    int reset_blocks;
-    int err;
-    char msg[256];
-};
-
-static void set_error(const arc_sim *cs, int code, const char *what, hipError_t e)
+void set_error(const arc_sim *cs, int code, const char *what, hipError_t e)
 {
     arc_sim *s = const_cast<arc_sim *>(cs);
     if (!s->err) {
@@ -63,22 +26,6 @@ static void set_error(const arc_sim *cs, int code, const char *what, hipError_t 
     fprintf(stderr, "libarcsim: %s: %s\n", what, e == hipSuccess ? "invalid argument" : hipGetErrorString(e));
 }
 
-#define HIPCHK(s, call, what)                                 \
-    do {                                                      \
-        hipError_t e_ = (call);                               \
-        if (e_ != hipSuccess) {                               \
-            set_error((s), (int)e_, (what), e_);              \
-            return;                                           \
-        }                                                     \
-    } while (0)
-#define HIPCHK_RET(s, call, what, ret)                        \
-    do {                                                      \
-        hipError_t e_ = (call);                               \
-        if (e_ != hipSuccess) {                               \
-            set_error((s), (int)e_, (what), e_);              \
-            return (ret);                                     \
-        }                                                     \
-    } while (0)
 
 static bool ensure_io(arc_sim *s, size_t doubles)
 {
@@ -130,11 +77,11 @@ static bool ensure_pin(arc_sim *s, size_t bytes)
 }
 
 /* the configured form of the step kernel over all worlds */
-static hipError_t launch_form(arc_sim *s, const double *td, const float *tf)
+hipError_t launch_form(arc_sim *s, const float *tf)
 {
     if (s->form == 1)
-        return launch_step_wave(s->dev, s->d_model, s->d_wave, td, tf, 0, s->stream);
-    return launch_step(s->dev, s->d_model, td, tf, nullptr, 0, s->n, s->block, s->grid_blocks, s->stream);
+        return launch_step_wave(s->dev, s->d_model, s->d_wave, nullptr, tf, 0, s->stream);
+    return launch_step(s->dev, s->d_model, tf, nullptr, s->n, s->block, s->tile, s->grid_blocks, nullptr, s->stream);
 }
 
 extern "C" {
@@ -166,6 +113,8 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
     s->cell_d = cell;
     s->block = 64;
     s->grid_blocks = 0;
+    s->tile = 16;
+    s->persistent = 0;
     (void)hipGetDevice(&s->device);
 
     if (!sim_setup(s->su, dt, substeps, grid, cell)) {
@@ -233,6 +182,7 @@ void arc_sim_destroy(arc_sim *s)
         return;
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
+    batch_free(s);
     dev_free(s->dev);
     if (s->d_model) (void)hipFree(s->d_model);
     if (s->d_reset) (void)hipFree(s->d_reset);
@@ -322,12 +272,16 @@ void arc_sim_step(arc_sim *s, const double *targets)
     size_t nt = (size_t)s->n * NJ;
     if (!nt)
         return;
-    if (!ensure_io(s, nt) || !ensure_pin(s, nt * sizeof(double)))
+    if (!ensure_io(s, nt) || !ensure_pin(s, nt * sizeof(float)))
         return;
     HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
-    memcpy(s->h_pin, targets, nt * sizeof(double));
-    HIPCHK(s, hipMemcpyAsync(s->d_io, s->h_pin, nt * sizeof(double), hipMemcpyHostToDevice, s->stream), "arc_sim_step upload");
-    HIPCHK(s, launch_form(s, s->d_io, nullptr), "arc_sim_step launch");
+    /* the engine reads binary32 targets; rounding before the clamp to binary32
+     * limits gives the clamp of the rounded value, as the FP32 engine computes */
+    float *hf = (float *)s->h_pin;
+    for (size_t k = 0; k < nt; ++k)
+        hf[k] = (float)targets[k];
+    HIPCHK(s, hipMemcpyAsync(s->d_io, s->h_pin, nt * sizeof(float), hipMemcpyHostToDevice, s->stream), "arc_sim_step upload");
+    HIPCHK(s, launch_form(s, (const float *)s->d_io), "arc_sim_step launch");
     HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
 }
 
@@ -506,7 +460,7 @@ int arcx_terrain(const arc_sim *cs, int i, double *out)
 
 int arcx_step_device(arc_sim *s, const float *targets_dev)
 {
-    HIPCHK_RET(s, launch_form(s, nullptr, targets_dev), "arcx_step_device", -1);
+    HIPCHK_RET(s, launch_form(s, targets_dev), "arcx_step_device", -1);
     return 0;
 }
 
@@ -525,7 +479,7 @@ int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long s
     HIPCHK_RET(s, hipEventRecord(t0, s->stream), "arcx_time_steps", -1);
     for (int k = 0; k < steps; ++k) {
         HIPCHK_RET(s, hipEventRecord(e0, s->stream), "arcx_time_steps", -1);
-        HIPCHK_RET(s, launch_form(s, nullptr, targets_dev + (size_t)k * step_floats), "arcx_time_steps", -1);
+        HIPCHK_RET(s, launch_form(s, targets_dev + (size_t)k * step_floats), "arcx_time_steps", -1);
         HIPCHK_RET(s, hipEventRecord(e1, s->stream), "arcx_time_steps", -1);
         HIPCHK_RET(s, hipEventSynchronize(e1), "arcx_time_steps", -1);
         float ms = 0.0f;
@@ -556,23 +510,14 @@ int arcx_task_config(arc_sim *s, int beached_steps, double flip_up_z, double voi
 }
 
 /* step, fall rules, auto-reset of the fallen worlds: no host round trip */
-static int step_task_impl(arc_sim *s, const float *targets_dev, const Range *ranges, int n_ranges, int total)
+int arcx_step_task(arc_sim *s, const float *targets_dev)
 {
     HIPCHK_RET(s, launch_fill_int(s->ts.count, 0, 1, s->stream), "arcx_step_task", -1);
-    if (ranges || s->form == 0)
-        HIPCHK_RET(s, launch_step(s->dev, s->d_model, nullptr, targets_dev, ranges, n_ranges, total, s->block,
-                                  s->grid_blocks, s->stream), "arcx_step_task step", -1);
-    else
-        HIPCHK_RET(s, launch_form(s, nullptr, targets_dev), "arcx_step_task step", -1);
-    HIPCHK_RET(s, launch_falls(s->dev, s->d_model, s->task, s->ts, s->stream), "arcx_step_task falls", -1);
+    HIPCHK_RET(s, launch_form(s, targets_dev), "arcx_step_task step", -1);
+    HIPCHK_RET(s, launch_falls(s->dev, s->d_model, s->task, s->ts, nullptr, s->stream), "arcx_step_task falls", -1);
     HIPCHK_RET(s, launch_reset(s->dev, s->d_model, s->d_reset, s->ts.list, s->ts.seeds, nullptr, s->ts.diff, 0,
                                s->ts.count, s->reset_blocks, s->stream), "arcx_step_task reset", -1);
     return 0;
-}
-
-int arcx_step_task(arc_sim *s, const float *targets_dev)
-{
-    return step_task_impl(s, targets_dev, nullptr, 0, s->n);
 }
 
 int arcx_task_causes(const arc_sim *cs, int *out)

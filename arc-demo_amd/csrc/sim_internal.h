@@ -1,0 +1,75 @@
+/*
+ * sim_internal.h -- the arc_sim handle shared by the host sources of
+ * libarcsim (sim.cpp: the arc_sim.h ABI; batch.cpp: batched, persistent and
+ * graph launches and the autotuner; server.cpp: the step server).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/arc_gpu.h"
+#include "device_state.h"
+#include "setup.h"
+#include "wave_tables.h"
+
+using namespace xpbd;
+
+struct BatchState;
+
+struct arc_sim {
+    int n;
+    int substeps;
+    double dt_d, cell_d;
+    SimSetup su;
+    DevWorlds dev;
+    Model *d_model;
+    ResetK *d_reset;
+    TemplateK *d_tmpl;
+    hipStream_t stream;
+    int device;
+    /* staging */
+    double *d_io;          /* device: max(N*12, N*64, N*k) doubles */
+    size_t d_io_doubles;
+    char *d_aux;           /* device: ids/seeds/difficulty/offsets */
+    size_t d_aux_bytes;
+    char *h_pin;           /* pinned host staging */
+    size_t h_pin_bytes;
+    float *d_dump;
+    arc_world *mirror;
+    int block, grid_blocks;
+    int tile;              /* LDS terrain tile side of the lane form: 0 or 16 */
+    int persistent;        /* lane form: wavefronts take work from a counter */
+    int form;              /* 0: one world per lane, 1: one world per wavefront */
+    WaveK wave;
+    WaveK *d_wave;
+    TaskK task;
+    TaskState ts;
+    void *ts_mem;
+    int reset_blocks;
+    int err;
+    char msg[256];
+    BatchState *batch;     /* batched / graph form (batch.cpp), created on first use */
+};
+
+/* records the first error of a handle (sticky) and prints it */
+void set_error(const arc_sim *cs, int code, const char *what, hipError_t e);
+/* the configured form of the step kernel over all worlds, float targets */
+hipError_t launch_form(arc_sim *s, const float *targets);
+/* releases the batch state of a handle */
+void batch_free(arc_sim *s);
+
+#define HIPCHK(s, call, what)                                 \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) {                               \
+            set_error((s), (int)e_, (what), e_);              \
+            return;                                           \
+        }                                                     \
+    } while (0)
+#define HIPCHK_RET(s, call, what, ret)                        \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) {                               \
+            set_error((s), (int)e_, (what), e_);              \
+            return (ret);                                     \
+        }                                                     \
+    } while (0)

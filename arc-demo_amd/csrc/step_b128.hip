@@ -1,0 +1,14 @@
+/* the step kernel with 128-thread workgroups (step_kernel.h) */
+#include "step_kernel.h"
+
+namespace xpbd {
+
+hipError_t launch_step_b128(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
+                          int total, int tile, int grid_blocks, int *work, hipStream_t s)
+{
+    return launch_step_block<128>(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+}
+
+int step_resident_blocks_b128(int tile) { return step_resident_blocks_of<128>(tile); }
+
+} /* namespace xpbd */

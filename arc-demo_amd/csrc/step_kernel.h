@@ -1,0 +1,284 @@
+/*
+ * step_kernel.h -- the lane-form step kernel (one world per lane, all
+ * substeps of a control step in one launch) and its launcher, instantiated
+ * once per workgroup size by step_b64.hip / step_b128.hip / step_b256.hip so
+ * the three compile in parallel.
+ *
+ * Work distribution, per launch:
+ *   plain      a grid-stride loop over worlds 0..total-1;
+ *   batched    the same over a StepBatch read from device memory at run time
+ *              (uneven world ranges of several requests; one captured graph
+ *              serves any batch the host writes into that table);
+ *   persistent a grid sized to the resident wavefronts, each wavefront taking
+ *              the next 64 worlds from an atomic counter until none are left.
+ * LDS layout, a template parameter: substep-start poses only (TILE = 0), or
+ * poses plus a TILE x TILE terrain tile per world.
+ */
+#pragma once
+#include "device_state.h"
+
+namespace xpbd {
+
+__device__ __forceinline__ void load_world(const DevWorlds &d, size_t i, LaneWorld &W)
+{
+    const size_t S = d.stride;
+    static_for<0, NB>([&](auto b) {
+        W.b[b].x = v3(d.x[(b * 3 + 0) * S + i], d.x[(b * 3 + 1) * S + i], d.x[(b * 3 + 2) * S + i]);
+        W.b[b].q = q4(d.q[(b * 4 + 0) * S + i], d.q[(b * 4 + 1) * S + i], d.q[(b * 4 + 2) * S + i],
+                      d.q[(b * 4 + 3) * S + i]);
+        W.b[b].v = v3(d.v[(b * 3 + 0) * S + i], d.v[(b * 3 + 1) * S + i], d.v[(b * 3 + 2) * S + i]);
+        W.b[b].w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
+    });
+    W.anchored = d.anchored[i];
+    W.unstable = d.unstable[i];
+    W.time = d.time[i];
+}
+
+__device__ __forceinline__ void store_world(const DevWorlds &d, size_t i, const LaneWorld &W)
+{
+    const size_t S = d.stride;
+    static_for<0, NB>([&](auto b) {
+        d.x[(b * 3 + 0) * S + i] = W.b[b].x.x; d.x[(b * 3 + 1) * S + i] = W.b[b].x.y; d.x[(b * 3 + 2) * S + i] = W.b[b].x.z;
+        d.q[(b * 4 + 0) * S + i] = W.b[b].q.w; d.q[(b * 4 + 1) * S + i] = W.b[b].q.x;
+        d.q[(b * 4 + 2) * S + i] = W.b[b].q.y; d.q[(b * 4 + 3) * S + i] = W.b[b].q.z;
+        d.v[(b * 3 + 0) * S + i] = W.b[b].v.x; d.v[(b * 3 + 1) * S + i] = W.b[b].v.y; d.v[(b * 3 + 2) * S + i] = W.b[b].v.z;
+        d.w[(b * 3 + 0) * S + i] = W.b[b].w.x; d.w[(b * 3 + 1) * S + i] = W.b[b].w.y; d.w[(b * 3 + 2) * S + i] = W.b[b].w.z;
+    });
+    static_for<0, NJ>([&](auto j) {
+        d.jout[(0 * NJ + j) * S + i] = W.angle[j];
+        d.jout[(1 * NJ + j) * S + i] = W.rate[j];
+        d.jout[(2 * NJ + j) * S + i] = W.torque[j];
+    });
+    d.anchored[i] = W.anchored;
+    static_for<0, NS>([&](auto s) { d.touch[s * S + i] = (uint8_t)touch_of(W, s); });
+    d.unstable[i] = W.unstable;
+    d.time[i] = W.time;
+}
+
+/* friction anchor of sphere s of world i, read and written in its HBM
+ * arrays only while the sphere is in contact */
+struct GlobalAnchor {
+    float *a, *n;
+    size_t S, i;
+    int s;
+    __device__ __forceinline__ V3 get(const float *p) const
+    {
+        return v3(p[(s * 3 + 0) * S + i], p[(s * 3 + 1) * S + i], p[(s * 3 + 2) * S + i]);
+    }
+    __device__ __forceinline__ void put(float *p, V3 v)
+    {
+        p[(s * 3 + 0) * S + i] = v.x;
+        p[(s * 3 + 1) * S + i] = v.y;
+        p[(s * 3 + 2) * S + i] = v.z;
+    }
+    __device__ __forceinline__ V3 point() const { return get(a); }
+    __device__ __forceinline__ V3 normal() const { return get(n); }
+    __device__ __forceinline__ void set_point(V3 v) { put(a, v); }
+    __device__ __forceinline__ void set_normal(V3 v) { put(n, v); }
+};
+struct GlobalAnchors {
+    float *a, *n;
+    size_t S, i;
+    __device__ __forceinline__ GlobalAnchor at(int s) const { return GlobalAnchor{ a, n, S, i, s }; }
+    __device__ __forceinline__ GlobalAnchors fresh() const { return GlobalAnchors{ a, n, S, launder(i) }; }
+};
+
+/* substep-start poses of the block's worlds in LDS: [body * 7 + field][thread] */
+template <int BLOCK> struct LdsPoses {
+    float *p;
+    int tid;
+    __device__ __forceinline__ void store(int b, const Pose0 &v)
+    {
+        float *q = p + b * 7 * BLOCK + tid;
+        q[0] = v.x0.x; q[BLOCK] = v.x0.y; q[2 * BLOCK] = v.x0.z;
+        q[3 * BLOCK] = v.q0.w; q[4 * BLOCK] = v.q0.x; q[5 * BLOCK] = v.q0.y; q[6 * BLOCK] = v.q0.z;
+    }
+    __device__ __forceinline__ Pose0 load(int b) const
+    {
+        const float *q = p + b * 7 * BLOCK + tid;
+        Pose0 v;
+        v.x0 = v3(q[0], q[BLOCK], q[2 * BLOCK]);
+        v.q0 = q4(q[3 * BLOCK], q[4 * BLOCK], q[5 * BLOCK], q[6 * BLOCK]);
+        return v;
+    }
+};
+
+/* The columns around a world's torso, copied into LDS once per control step:
+ * TILE x TILE int8 heights of the padded grid, [column][thread].  Every lookup
+ * inside the tile reads LDS; a lookup outside it (a leg stretched beyond the
+ * tile, or a world that left the grid) reads the grid in HBM, which holds the
+ * same bytes, so the heights are identical either way. */
+template <int BLOCK, int TILE> struct LdsTileCells {
+    const TerrainK &t;
+    const TerrainRef &r;
+    const int8_t *lt;
+    int tx, ty;
+    __device__ __forceinline__ float operator()(int ix, int iy) const
+    {
+        unsigned u = (unsigned)(ix - tx), v = (unsigned)(iy - ty);
+        if (u < (unsigned)TILE && v < (unsigned)TILE)
+            return dequant(r, lt[(u + v * TILE) * BLOCK]);
+        return terrain_cell(t, r, ix, iy);
+    }
+};
+template <int BLOCK, int TILE> struct LdsTile {
+    const int8_t *lt;
+    int tx, ty;
+    __device__ __forceinline__ LdsTileCells<BLOCK, TILE> cells(const TerrainK &t, const TerrainRef &r) const
+    {
+        return LdsTileCells<BLOCK, TILE>{ t, r, lt, tx, ty };
+    }
+};
+
+/* fill this thread's tile of world i around the torso position (x, y) */
+template <int BLOCK, int TILE>
+__device__ __forceinline__ LdsTile<BLOCK, TILE> fill_tile(int8_t *s_tile, const DevWorlds &d, const TerrainK &t,
+                                                          size_t i, float x, float y)
+{
+    LdsTile<BLOCK, TILE> T;
+    int8_t *lt = s_tile + threadIdx.x;
+    T.lt = lt;
+    const int n = d.tpitch - 2 * TPAD;
+    if (d.tpitch < TILE) {
+        T.tx = T.ty = 1 << 28; /* grid smaller than a tile: every lookup goes to HBM */
+        return T;
+    }
+    /* clamped in float first: a non-finite torso still yields a valid tile */
+    float fx = fminf(fmaxf((x - t.ox) / t.cell, -1.0e4f), 1.0e4f);
+    float fy = fminf(fmaxf((y - t.oy) / t.cell, -1.0e4f), 1.0e4f);
+    T.tx = min(max((int)floorf(fx) - TILE / 2, -TPAD), n + TPAD - TILE);
+    T.ty = min(max((int)floorf(fy) - TILE / 2, -TPAD), n + TPAD - TILE);
+    const int8_t *src = d.tq + i * d.tcells + (T.tx + TPAD) + (size_t)(T.ty + TPAD) * d.tpitch;
+#pragma unroll 2
+    for (int v = 0; v < TILE; ++v) {
+#pragma unroll
+        for (int w = 0; w < TILE / 4; ++w) {
+            uint32_t word;
+            __builtin_memcpy(&word, src + v * d.tpitch + 4 * w, 4);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                lt[(4 * w + b + v * TILE) * BLOCK] = (int8_t)(word >> (8 * b));
+        }
+    }
+    return T;
+}
+
+/* An index the optimiser cannot see through.  The world's SoA addresses are
+ * recomputed from it for the stores at the end of the step instead of being
+ * shared with the loads at its start, which would keep ~450 64-bit addresses
+ * live across the whole step. */
+__device__ __forceinline__ size_t opaque_index(size_t i)
+{
+    asm volatile("" : "+v"(i));
+    return i;
+}
+
+/* one control step of world i by the calling lane */
+template <int BLOCK, int TILE>
+__device__ __forceinline__ void step_world(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+                                           LdsPoses<BLOCK> &P0, int8_t *s_tile, int i)
+{
+    const size_t S = d.stride;
+    LaneWorld W;
+    /* targets clamped to the limits, csrc/arc_sim.c:132-134 */
+    static_for<0, NJ>([&](auto j) {
+        float tg = targets[(size_t)i * NJ + j];
+        W.target[j] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
+        d.target[j * S + i] = W.target[j];
+    });
+    if (d.unstable[i])
+        return; /* frozen until reset, csrc/arc_sim.c:135-136 */
+    GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
+    tr.outside = d.outside[i];
+    ContactStore cs;
+    cs.base = d.cbuf;
+    cs.stride = S;
+    cs.world = i;
+    int ov;
+    if constexpr (TILE > 0) {
+        const LdsTile<BLOCK, TILE> tile =
+            fill_tile<BLOCK, TILE>(s_tile, d, M.terr, (size_t)i, d.x[0 * S + i], d.x[1 * S + i]);
+        load_world(d, i, W);
+        ov = lane_step(W, A, P0, M, tr, cs, tile);
+    } else {
+        load_world(d, i, W);
+        ov = lane_step(W, A, P0, M, tr, cs);
+    }
+    const size_t io = opaque_index((size_t)i);
+    store_world(d, io, W);
+    if (ov)
+        d.err[io] |= ov;
+}
+
+template <int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__restrict__ m,
+                                                const float *__restrict__ targets, const StepBatch *__restrict__ batch,
+                                                int total, int *__restrict__ work)
+{
+    __shared__ float s_pose[NB * 7 * BLOCK];
+    __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * BLOCK : 1];
+    const Model &M = *m;
+    LdsPoses<BLOCK> P0;
+    P0.p = s_pose;
+    P0.tid = threadIdx.x;
+    const int n_ranges = batch ? batch->n_ranges : 0;
+    if (batch)
+        total = batch->total;
+    const int lane = threadIdx.x & 63;
+    /* each wavefront steps 64 consecutive work items at a time: the next
+     * ones from the counter (persistent) or one grid further on */
+    auto next = [&](int base) {
+        if (!work)
+            return base + (int)gridDim.x * BLOCK;
+        int v = 0;
+        if (lane == 0)
+            v = atomicAdd(work, 64);
+        return __shfl(v, 0, 64);
+    };
+    int base = work ? next(0) : (int)blockIdx.x * BLOCK + (threadIdx.x & ~63);
+    while (base < total) {
+        const int t = base + lane;
+        if (t < total)
+            step_world<BLOCK, TILE>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, t) : t);
+        base = next(base);
+    }
+}
+
+/* workgroups of the BLOCK-thread kernel resident on the whole device at once */
+template <int BLOCK>
+int step_resident_blocks_of(int tile)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    hipError_t e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 16>, BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 0>, BLOCK, 0);
+    if (e != hipSuccess)
+        return 0;
+    return per * cus;
+}
+
+/* launcher of the BLOCK-thread kernels (grid_blocks 0: one item per thread) */
+template <int BLOCK>
+hipError_t launch_step_block(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
+                             int total, int tile, int grid_blocks, int *work, hipStream_t s)
+{
+    if (grid_blocks <= 0)
+        grid_blocks = (total + BLOCK - 1) / BLOCK;
+    if (grid_blocks <= 0)
+        return hipSuccess;
+    if (tile == 16)
+        hipLaunchKernelGGL((k_step<BLOCK, 16>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch, total,
+                           work);
+    else if (tile == 0)
+        hipLaunchKernelGGL((k_step<BLOCK, 0>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch, total,
+                           work);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+} /* namespace xpbd */

@@ -87,6 +87,40 @@ double arcx_bytes_per_world(const arc_sim *s);
 /* Contact-list overflow flags of world i (0 expected). */
 int arcx_world_error(const arc_sim *s, int i);
 
+/* ---- batched, persistent and graph form (lane form only) -------------
+ * No counterpart in the reference, whose arc_sim_step always steps every
+ * world (csrc/arc_sim.c:123-138); these serve callers that step several
+ * independent requests' worlds together. */
+
+/* LDS layout of the lane form: tile = side of the per-world terrain tile held
+ * in LDS (16), or 0 for substep-start poses only (terrain read from HBM). */
+int arcx_set_layout(arc_sim *s, int tile);
+/* 1: persistent distribution, a grid of resident workgroups whose
+ * wavefronts take 64 worlds at a time from a device counter. */
+int arcx_set_persistent(arc_sim *s, int on);
+/* out4 = threads per block, tile, blocks (persistent: resident blocks), persistent */
+int arcx_get_launch(const arc_sim *s, int *out4);
+
+/* One control step of the worlds [starts[r], starts[r] + counts[r]) of n_req
+ * (<= 256) requests in one launch.  targets_dev: device float N x 12,
+ * world-major (rows of worlds not in the batch are not read).  Ranges must be
+ * inside [0, N) and disjoint (checked; -1 otherwise).  task = 1 adds the fall
+ * rules and auto-reset of the stepped worlds (as arcx_step_task).  graph = 1
+ * replays a captured hipGraph of the launch sequence; it is captured again
+ * only when targets_dev, task or the launch configuration change, while the
+ * request table is read from device memory at every replay.  Asynchronous;
+ * arcx_sync waits. */
+int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
+                    int graph);
+/* graphs captured so far by arcx_step_batch */
+int arcx_graph_captures(const arc_sim *s);
+
+/* Autotuner: times every (threads per block, LDS layout, distribution) of
+ * the lane form over `steps` control steps of this handle's own worlds from
+ * their current state (restored afterwards), keeps the fastest and reports it
+ * in out4 (as arcx_get_launch) and its ms per step in ms_out. */
+int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,158 @@
+"""GPU batched form: uneven request ranges in one launch, the persistent
+work-queue distribution, hipGraph replays and the autotuner.
+
+Every launch configuration runs the same per-world arithmetic, so the bar
+here is bit-identity with the library's plain full-sim step (itself held to
+the oracle by test_gpu_parity.py), plus the oracle directly for one batch."""
+
+import numpy as np
+import pytest
+
+import pyoracle as P
+from helpers import action_stream
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def native():
+    from arc import native
+    return native
+
+
+def _sim(native, n, seed0=0):
+    g = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, (ids + seed0).astype(np.uint32), P.difficulties(1, n))
+    return g
+
+
+def _bits(sim, n):
+    return np.stack([sim.state(i) for i in range(n)]).view(np.uint64)
+
+
+def test_batch_steps_only_its_ranges(native):
+    n = 200
+    ranges = [(0, 5), (10, 27), (100, 64), (199, 1)]
+    t = action_stream(n, 1)[0].astype(np.float32)
+    dev = native.DeviceBuffer.from_array(t)
+    g = _sim(native, n)
+    before = _bits(g, n)
+    g.step_batch([r[0] for r in ranges], [r[1] for r in ranges], dev.ptr)
+    g.sync()
+    after = _bits(g, n)
+    full = _sim(native, n)
+    full.step_device(dev.ptr)
+    full.sync()
+    ref = _bits(full, n)
+    inside = np.zeros(n, bool)
+    for s, c in ranges:
+        inside[s:s + c] = True
+    assert np.array_equal(after[inside], ref[inside])
+    assert np.array_equal(after[~inside], before[~inside])
+    # and the stepped worlds against the oracle
+    o = P.Lib("o32").sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    o.reset(ids, ids.astype(np.uint32), P.difficulties(1, n))
+    o.step(t.astype(np.float64))
+    a = after[inside].view(np.float64)
+    b = np.stack([o.state(i) for i in np.flatnonzero(inside)])
+    assert np.all(np.abs(a - b) <= 1e-5 * (np.abs(b) + 1.0))
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_launch_configurations_bit_identical(native, block):
+    n, steps = 300, 2
+    ts = action_stream(n, steps).astype(np.float32)
+    dev = native.DeviceBuffer.from_array(ts)
+    per = n * 12 * 4
+    base = _sim(native, n)
+    for k in range(steps):
+        base.step_device(dev.ptr + k * per)
+    base.sync()
+    ref = _bits(base, n)
+    for tile in (0, 16):
+        for persistent in (False, True):
+            g = _sim(native, n)
+            g.set_launch(block, 0)
+            g.set_layout(tile)
+            g.set_persistent(persistent)
+            assert g.launch()["block"] == block and g.launch()["tile"] == tile
+            for k in range(steps):
+                g.step_batch([0], [n], dev.ptr + k * per)
+            g.sync()
+            assert np.array_equal(_bits(g, n), ref), f"block {block} tile {tile} persistent {persistent}"
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_graph_replays_match_eager(native, persistent):
+    """One captured graph, replayed with a different request table each step
+    (and new target values uploaded between replays), against eager launches
+    of the same batches; eager read-backs between the replays."""
+    n = 256
+    batches = [([0, 64], [32, 100]), ([5], [251]), ([0, 128, 200], [10, 60, 56]), ([0], [256]), ([77], [3])]
+    t = native.DeviceBuffer(n * 12 * 4)
+    gr, eg = _sim(native, n), _sim(native, n)
+    for s in (gr, eg):
+        s.set_persistent(persistent)
+    for k, (starts, counts) in enumerate(batches):
+        t.upload(action_stream(n, k + 1, seed=5)[k].astype(np.float32))
+        gr.step_batch(starts, counts, t.ptr, graph=True)
+        eg.step_batch(starts, counts, t.ptr, graph=False)
+        gr.sync()
+        eg.sync()
+        assert np.array_equal(_bits(gr, n), _bits(eg, n)), f"replay {k}"
+    assert gr.graph_captures() == 1
+
+
+def test_batch_task_matches_step_task(native):
+    """Full-range batches with the task (fall rules + auto-reset) captured in a
+    graph against arcx_step_task, under tightened rules that terminate many
+    worlds; and a partial batch applies the rules to its own worlds only."""
+    n, steps = 128, 5
+    ts = action_stream(n, steps, seed=3).astype(np.float32)
+    dev = native.DeviceBuffer.from_array(ts)
+    per = n * 12 * 4
+    a, b = _sim(native, n), _sim(native, n)
+    for s in (a, b):
+        s.task_config(2, 0.97, -2.0)
+    for k in range(steps):
+        a.step_batch([0], [n], dev.ptr + k * per, task=True, graph=True)
+        b.step_task(dev.ptr + k * per)
+        a.sync()
+        b.sync()
+        assert np.array_equal(a.task_causes(), b.task_causes()), f"step {k}"
+        assert np.array_equal(_bits(a, n), _bits(b, n)), f"step {k}"
+    assert a.task_stats()[1:].sum() > 10
+    assert np.array_equal(a.task_episodes(), b.task_episodes())
+    # partial batch: worlds outside it keep their cause 0 and episode count
+    c = _sim(native, n)
+    c.task_config(2, 0.97, -2.0)
+    for k in range(steps):
+        c.step_batch([0], [n // 2], dev.ptr + k * per, task=True)
+    c.sync()
+    assert np.all(c.task_causes()[n // 2:] == 0) and np.all(c.task_episodes()[n // 2:] == 0)
+    assert c.task_episodes()[: n // 2].sum() > 0
+
+
+def test_autotune_restores_state(native):
+    n = 512
+    g = _sim(native, n)
+    before = _bits(g, n)
+    cfg, ms = g.autotune(2)
+    assert np.array_equal(_bits(g, n), before)
+    assert cfg["block"] in (64, 128, 256) and cfg["tile"] in (0, 16) and ms > 0.0
+    assert g.launch() == cfg
+    print(f"autotune at {n} worlds: {cfg}, {ms:.3f} ms/step")
+
+
+def test_batch_rejects_bad_requests(native):
+    n = 64
+    g = _sim(native, n)
+    dev = native.DeviceBuffer(n * 12 * 4)
+    with pytest.raises(native.SimError):
+        g.step_batch([0, 10], [20, 5], dev.ptr)  # overlap
+    with pytest.raises(native.SimError):
+        g.step_batch([60], [5], dev.ptr)  # past the end
+    with pytest.raises(native.SimError):
+        g.step_batch(list(range(257)), [0] * 257, dev.ptr)  # too many requests
