@@ -28,7 +28,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
           "-fno-gpu-rdc"]
 
 HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip"]
-HOST_SOURCES = ["sim.cpp", "batch.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp"]
+HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp"]
 
 
 def hipcc() -> str:
@@ -72,7 +72,8 @@ def build(jobs: int = 4) -> Path:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda t: compile_one(*t), tasks))
     tmp = OUT.with_suffix(".so.tmp")
-    _run([hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", str(tmp)] + [str(o) for o in objs])
+    _run([hipcc(), "-shared", "-pthread", "-Wl,--no-undefined", f"--offload-arch={ARCH}", "-o", str(tmp)]
+         + [str(o) for o in objs])
     os.replace(tmp, OUT)
     return OUT
 

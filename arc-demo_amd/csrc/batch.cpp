@@ -21,7 +21,8 @@
 
 namespace {
 
-constexpr int RING = 4; /* pinned request tables in flight */
+constexpr int RING = 4;   /* pinned request tables in flight */
+constexpr int GRAPHS = 4; /* captured graphs kept per handle */
 
 struct LaunchKey {
     const float *targets;
@@ -41,9 +42,14 @@ struct BatchState {
     hipEvent_t ev[RING] = {};
     int slot = 0;
     int *d_work = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    LaunchKey key{};
+    /* captured graphs, one per launch key, least recently used replaced */
+    struct Entry {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        LaunchKey key{};
+        long long used = 0;
+    } cache[GRAPHS];
+    long long clock = 0;
     int captures = 0;
 };
 
@@ -52,10 +58,12 @@ void batch_free(arc_sim *s)
     BatchState *b = s->batch;
     if (!b)
         return;
-    if (b->exec)
-        (void)hipGraphExecDestroy(b->exec);
-    if (b->graph)
-        (void)hipGraphDestroy(b->graph);
+    for (auto &c : b->cache) {
+        if (c.exec)
+            (void)hipGraphExecDestroy(c.exec);
+        if (c.graph)
+            (void)hipGraphDestroy(c.graph);
+    }
     for (int k = 0; k < RING; ++k) {
         if (b->ev[k])
             (void)hipEventDestroy(b->ev[k]);
@@ -124,30 +132,57 @@ static hipError_t enqueue(arc_sim *s, BatchState *b, const float *targets, int t
                         s->reset_blocks, s->stream);
 }
 
-static hipError_t capture(arc_sim *s, BatchState *b, const LaunchKey &k)
+/* the graph of key k, captured into the least recently used slot if absent */
+static hipError_t graph_for(arc_sim *s, BatchState *b, const LaunchKey &k, hipGraphExec_t *out)
 {
-    if (b->exec) {
-        (void)hipGraphExecDestroy(b->exec);
-        b->exec = nullptr;
+    BatchState::Entry *e = nullptr;
+    for (auto &c : b->cache)
+        if (c.exec && c.key == k)
+            e = &c;
+    if (!e) {
+        e = &b->cache[0];
+        for (auto &c : b->cache)
+            if (c.used < e->used)
+                e = &c;
+        if (e->exec)
+            (void)hipGraphExecDestroy(e->exec);
+        if (e->graph)
+            (void)hipGraphDestroy(e->graph);
+        e->exec = nullptr;
+        e->graph = nullptr;
+        hipError_t r = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (r != hipSuccess)
+            return r;
+        hipError_t r1 = enqueue(s, b, k.targets, k.task, k.grid);
+        hipError_t r2 = hipStreamEndCapture(s->stream, &e->graph);
+        if (r1 != hipSuccess)
+            return r1;
+        if (r2 != hipSuccess)
+            return r2;
+        if ((r = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0)) != hipSuccess)
+            return r;
+        e->key = k;
+        b->captures++;
     }
-    if (b->graph) {
-        (void)hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
-    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    e->used = ++b->clock;
+    *out = e->exec;
+    return hipSuccess;
+}
+
+/* every world of the sim, lane form, in the configured distribution */
+hipError_t launch_lane_all(arc_sim *s, const float *targets)
+{
+    if (!s->persistent)
+        return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, s->grid_blocks, nullptr,
+                           s->stream);
+    BatchState *b = batch_of(s);
+    if (!b)
+        return hipErrorOutOfMemory;
+    hipError_t e = launch_fill_int(b->d_work, 0, 1, s->stream);
     if (e != hipSuccess)
         return e;
-    hipError_t e1 = enqueue(s, b, k.targets, k.task, k.grid);
-    hipError_t e2 = hipStreamEndCapture(s->stream, &b->graph);
-    if (e1 != hipSuccess)
-        return e1;
-    if (e2 != hipSuccess)
-        return e2;
-    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
-        return e;
-    b->key = k;
-    b->captures++;
-    return hipSuccess;
+    return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, grid_for(s, s->n), b->d_work,
+                       s->stream);
 }
 
 extern "C" {
@@ -237,9 +272,9 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
         return 0;
     }
     LaunchKey k{ targets_dev, task ? 1 : 0, s->block, s->tile, grid_for(s, std::max(s->n, 1)), s->persistent };
-    if (!b->exec || !(k == b->key))
-        HIPCHK_RET(s, capture(s, b, k), "arcx_step_batch capture", -1);
-    HIPCHK_RET(s, hipGraphLaunch(b->exec, s->stream), "arcx_step_batch graph", -1);
+    hipGraphExec_t exec = nullptr;
+    HIPCHK_RET(s, graph_for(s, b, k, &exec), "arcx_step_batch capture", -1);
+    HIPCHK_RET(s, hipGraphLaunch(exec, s->stream), "arcx_step_batch graph", -1);
     return 0;
 }
 

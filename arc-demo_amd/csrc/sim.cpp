@@ -81,7 +81,7 @@ hipError_t launch_form(arc_sim *s, const float *tf)
 {
     if (s->form == 1)
         return launch_step_wave(s->dev, s->d_model, s->d_wave, nullptr, tf, 0, s->stream);
-    return launch_step(s->dev, s->d_model, tf, nullptr, s->n, s->block, s->tile, s->grid_blocks, nullptr, s->stream);
+    return launch_lane_all(s, tf);
 }
 
 extern "C" {

@@ -56,6 +56,8 @@ void set_error(const arc_sim *cs, int code, const char *what, hipError_t e);
 hipError_t launch_form(arc_sim *s, const float *targets);
 /* releases the batch state of a handle */
 void batch_free(arc_sim *s);
+/* every world of the sim, lane form, in the configured distribution */
+hipError_t launch_lane_all(arc_sim *s, const float *targets);
 
 #define HIPCHK(s, call, what)                                 \
     do {                                                      \

@@ -1,27 +1,43 @@
 """Benchmark of the batched XPBD world step on one MI355X.
 
-Workload (the metric's configuration): 65,536 worlds of the quadruped, each on
-block terrain of per-world difficulty U[0, 1) drawn from the engine RNG seeded
-1, world i reset with seed i; dt = 1/30 s, 20 substeps, grid 64 x 0.4 m;
-actions i.i.d. uniform [-1, 1) from the engine RNG seeded 0 (world-major,
-joint-minor), target = default + 0.6 a.  Targets for every timed step are
-generated before the timed region and are resident in HBM.
+Headline (value): the served form at 65,536 worlds -- 16 clients of 4,096
+worlds each in one device-resident step server, each client keeping two step
+requests in flight (the closed-loop figure, one in flight, is reported beside
+it), every world on block terrain
+of per-world difficulty U[0, 1) (engine RNG), world i reset with seed i,
+dt = 1/30 s, 20 substeps, grid 64 x 0.4 m, actions i.i.d. uniform [-1, 1)
+from the engine RNG (target = default + 0.6 a), fall rules and auto-reset on.
+A "step" is one control step of all 65,536 worlds (one request of every
+client); value = 65,536 x steps / seconds over exactly --steps timed steps
+after --warmup untimed ones.
 
-A "step" is one control step of every world (20 XPBD substeps each);
-value = worlds x steps / seconds over exactly --steps timed steps.
+Beside it, in the same run:
+  batched   the same worlds as one sim stepped by the batched launch
+            (autotuned configuration) replayed from a hipGraph, resets on;
+  plain     the plain form, one world per wavefront, one launch per step;
+  latency   4,096 worlds at difficulty 0.5, seeds 0..4095, batched graph with
+            resets: p50 / p99 of the per-step latency (host clock around
+            launch + completion);
+  serving   16 clients with world counts floor(exp(U ln 8192)), Poisson step
+            arrivals at half the measured saturation rate: p50 / p99 request
+            latency and throughput;
+  roofline  the dominant kernel (k_step) timed with HIP events on the
+            engine's stream over --steps launches at 65,536 worlds;
+  cpu       the reference engine built from its sources with its shipped
+            flags (oracle/_ref), on this host's cores, bounded sample.
 
-Prints ONE JSON line (rank 0) with the roofline of the dominant kernel
-(k_step) and the CPU baseline: the reference engine built from its own
-sources with its shipped flags (oracle/_ref), timed on this host's cores on a
-bounded sample of the same workload.
+Prints ONE JSON line.  `--only kernel` runs just the kernel timing pass (for
+counter collection under rocprofv3).
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import os
+import math
+import re
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -31,92 +47,214 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "arc-demo_amd"))
 
 METRIC = "world-steps/s @65536 worlds (served vs plain GPU vs host CPU); p50/p99 step lat"
-FLOP_PER_WORLD_STEP = 4.12e5     # algorithmic FLOP per world-step (op count of the reference)
+FLOP_PER_WORLD_STEP = 4.12e5     # algorithmic FLOP per world-step (op count of the reference engine)
 BYTES_PER_WORLD_STEP = 2.8e3     # algorithmic HBM bytes per world-step (state in/out, targets, outputs)
-PEAK_FP32 = 157.3e12             # MI355X vector FP32, FLOP/s
+PEAK_FP32 = 157.3e12             # MI355X FP32 vector peak, FLOP/s
 PEAK_HBM = 8.0e12                # MI355X HBM3E, B/s
 NJ = 12
+N = 65536
+CLIENTS = 16
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_step_lane_65536w_pmc.md"
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--block", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--cpu-worlds", type=int, default=4096)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "worlds shard by HBM budget across GPUs with no exchange inside a step", file=sys.stderr)
-        return 2
-
-    from arc import native
-
-    n = args.worlds
-    steps, warm = args.steps, args.warmup
-    sim = native.Sim(n)
-    sim.set_launch(args.block, 0)
-    # synthetic inputs: difficulty per world, actions per step
-    diff_u = _rng_stream(1, n)
-    diff = (diff_u >> 8).astype(np.float64) * (1.0 / 16777216.0)
-    ids = np.arange(n, dtype=np.int32)
-    sim.reset(ids, ids.astype(np.uint32), diff)
-    u = _rng_stream(0, n * NJ * (steps + warm))
-    a = (u >> 8).astype(np.float64) * (1.0 / 8388608.0) - 1.0
-    targets = (np.tile(sim.default_angles, n * (steps + warm))[: a.size] + 0.6 * a).astype(np.float32)
-    dev = native.DeviceBuffer.from_array(targets)
-    base = dev.ptr
-    per = n * NJ
-    # warmup
-    if warm:
-        sim.time_steps(base, warm, per)
-    sim.sync()
-    t0 = time.perf_counter()
-    kernel_ms, wall_ms = sim.time_steps(base + 4 * per * warm, steps, per)
-    sim.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = n * steps / secs
-    avg_kernel_s = kernel_ms / 1e3 / steps
-    achieved = n * FLOP_PER_WORLD_STEP / avg_kernel_s
-    flags = sim.observe()[3]
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "world-steps/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": warm,
-        "ms_per_step": 1e3 * secs / steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{n} worlds, per-world difficulty U[0,1), uniform random actions, resets off",
-                   "worlds": n, "dt": 1.0 / 30.0, "substeps": 20, "grid": 64, "cell": 0.4,
-                   "form": "per-step launch, one world per lane", "block": args.block},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "hbm_frac": n * BYTES_PER_WORLD_STEP / avg_kernel_s / PEAK_HBM,
-                     "kernel": "k_step", "kernel_ms_avg": 1e3 * avg_kernel_s},
-        "unstable_worlds": int(flags.sum()),
-        "bytes_per_world": sim.bytes_per_world(),
-    }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
-    return 0
-
-
-def _rng_stream(seed: int, n: int) -> np.ndarray:
-    """The engine's xorshift128 stream (libarcsim's own generator)."""
+def rng(seed: int, n: int) -> np.ndarray:
     from arc import native
     return native.rng_stream(seed, n)
+
+
+def unit24(u: np.ndarray) -> np.ndarray:
+    return (u >> 8).astype(np.float64) * (1.0 / 16777216.0)
+
+
+def actions_to_targets(defaults: np.ndarray, u: np.ndarray) -> np.ndarray:
+    a = (u >> 8).astype(np.float64) * (1.0 / 8388608.0) - 1.0
+    return (np.tile(defaults, u.size // NJ) + 0.6 * a).astype(np.float32)
+
+
+def new_sim(n: int, difficulty=None):
+    from arc import native
+    s = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    d = unit24(rng(1, n)) if difficulty is None else np.full(n, difficulty)
+    s.reset(ids, ids.astype(np.uint32), d)
+    return s
+
+
+def kernel_pass(steps: int, warm: int) -> dict:
+    """k_step at 65,536 worlds: HIP events around each launch on the engine stream."""
+    from arc import native
+    s = new_sim(N)
+    cfg, _ = s.autotune(3)
+    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+    per = N * NJ
+    for k in range(warm):
+        s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+    s.sync()
+    kms = 0.0
+    for k in range(steps):
+        m, _ = s.time_steps(t.ptr + 4 * per * (k % 8), 1, per)
+        kms += m
+    avg_s = kms / 1e3 / steps
+    return {"launch": cfg, "kernel_ms_avg": 1e3 * avg_s, "launches": steps}
+
+
+def batched_pass(steps: int, warm: int) -> dict:
+    from arc import native
+    s = new_sim(N)
+    cfg, _ = s.autotune(3)
+    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+    per = N * NJ
+    for k in range(warm):
+        s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+    s.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step_batch([0], [N], t.ptr + 4 * per * ((warm + k) % 8), task=True, graph=True)
+    s.sync()
+    el = time.perf_counter() - t0
+    stats = s.task_stats()
+    return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "launch": cfg,
+            "resets": int(stats[1:].sum()), "bytes_per_world": s.bytes_per_world()}
+
+
+def plain_pass(steps: int = 3) -> dict:
+    from arc import native
+    s = new_sim(N)
+    s.set_form("wave")
+    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 2)))
+    per = N * NJ
+    s.step_task(t.ptr)
+    s.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step_task(t.ptr + 4 * per * ((k + 1) % 2))
+    s.sync()
+    el = time.perf_counter() - t0
+    return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+            "form": "one world per wavefront, world staged in LDS, one launch per step"}
+
+
+def latency_pass(steps: int, warm: int) -> dict:
+    from arc import native
+    n = 4096
+    s = new_sim(n, 0.5)
+    cfg, _ = s.autotune(3)
+    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, n * NJ * 8)))
+    per = n * NJ
+    lat = []
+    for k in range(warm + steps):
+        t0 = time.perf_counter()
+        s.step_batch([0], [n], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+        s.sync()
+        if k >= warm:
+            lat.append(1e3 * (time.perf_counter() - t0))
+    lat = np.array(lat)
+    return {"worlds": n, "difficulty": 0.5, "steps": steps, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "value": n / (lat.mean() / 1e3), "launch": cfg}
+
+
+def served_pass(steps: int, warm: int) -> dict:
+    """16 clients x 4,096 worlds; a step = one request of every client.
+    Closed loop (each client waits for its step before sending the next) and
+    pipelined (each client keeps two requests in flight)."""
+    from arc import native, server
+    per_world = native.Sim(1).bytes_per_world() + 2 * NJ * 4
+    sv = server.StepServer(N * per_world * 1.0001)
+    n = N // CLIENTS
+    cids = [sv.open(n, seed0=c * n, difficulty=-1.0) for c in range(CLIENTS)]
+    defaults = native.Sim(1).default_angles
+    acts = [[actions_to_targets(defaults, rng(1000 + 8 * c + j, n * NJ)) for j in range(4)] for c in range(CLIENTS)]
+
+    def run(k0, count, depth):
+        inflight = []
+        for k in range(k0, k0 + count):
+            inflight.append([sv.submit(c, acts[c][k % 4]) for c in cids])
+            if len(inflight) >= depth:
+                for x in inflight.pop(0):
+                    sv.wait(x)
+        for ts in inflight:
+            for x in ts:
+                sv.wait(x)
+
+    out = {"clients": CLIENTS, "worlds_per_client": n}
+    k = 0
+    for depth, name in ((1, "closed_loop"), (2, "pipelined")):
+        run(k, warm, depth)
+        k += warm
+        sv.reset_stats()
+        t0 = time.perf_counter()
+        run(k, steps, depth)
+        el = time.perf_counter() - t0
+        k += steps
+        st = sv.stats()
+        out[name] = {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "requests_in_flight_per_client": depth,
+                     "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "requests_per_batch": st["requests_per_batch"]}
+    sv.close_server()
+    out["value"] = out["pipelined"]["value"]
+    out["ms_per_step"] = out["pipelined"]["ms_per_step"]
+    return out
+
+
+def serving_pass(seconds: float) -> dict:
+    """Clients log-uniform in [1, 8192] worlds, Poisson arrivals at half the
+    saturation rate measured first in closed loop."""
+    from arc import native, server
+    u = rng(4, CLIENTS).astype(np.float64) * (1.0 / 4294967296.0)
+    sizes = [max(1, int(math.floor(math.exp(x * math.log(8192.0))))) for x in u]
+    per_world = native.Sim(1).bytes_per_world() + 2 * NJ * 4
+    sv = server.StepServer(sum(sizes) * per_world * 1.0001)
+    cids = [sv.open(n, seed0=10000 * c, difficulty=-1.0) for c, n in enumerate(sizes)]
+    defaults = native.Sim(1).default_angles
+    acts = [actions_to_targets(defaults, rng(2000 + c, n * NJ)) for c, n in enumerate(sizes)]
+    # saturation: every client keeps one request in flight
+    stop = time.perf_counter() + min(1.0, seconds / 3)
+    for c in cids:
+        sv.wait(sv.submit(c, acts[c]))
+    sv.reset_stats()
+
+    def closed(c):
+        while time.perf_counter() < stop:
+            sv.wait(sv.submit(c, acts[c]))
+
+    th = [threading.Thread(target=closed, args=(c,)) for c in cids]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    sat = sv.stats()
+    rate = 0.5 * sat["requests"] / sat["seconds"]
+    # open loop: merged Poisson arrivals, each to a uniformly chosen client
+    g = np.random.default_rng(5)
+    gaps = g.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
+    when = np.cumsum(gaps)
+    when = when[when < seconds]
+    who = g.integers(0, CLIENTS, size=when.size)
+    sv.reset_stats()
+    tickets = []
+    t0 = time.perf_counter()
+    for w, c in zip(when, who):
+        d = t0 + w - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(sv.submit(cids[c], acts[c]))
+    for x in tickets:
+        sv.wait(x)
+    st = sv.stats()
+    sv.close_server()
+    return {"clients": CLIENTS, "worlds": int(sum(sizes)), "sizes": sizes,
+            "saturation_requests_per_s": sat["requests"] / sat["seconds"],
+            "saturation_world_steps_per_s": sat["world_steps_per_s"], "offered_requests_per_s": rate,
+            "requests": int(st["requests"]), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+            "world_steps_per_s": st["world_steps_per_s"], "requests_per_batch": st["requests_per_batch"]}
+
+
+def traffic_from_profile():
+    """HBM bytes per k_step launch from the committed counter summary (or None)."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_PROFILE.read_text())
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(n: int, seconds: float) -> dict:
@@ -132,26 +270,117 @@ def cpu_baseline(n: int, seconds: float) -> dict:
     threads = min(16, P.cpu_threads())
     L = P.Lib(kind)
     s = L.sim(n)
-    d = (_rng_stream(1, n) >> 8).astype(np.float64) * (1.0 / 16777216.0)
+    d = unit24(rng(1, n))
     ids = np.arange(n, dtype=np.int32)
     s.reset(ids, ids.astype(np.uint32), d)
     defaults = s.robot_info()[0]
-    rs = _rng_stream(0, n * NJ * 64)
-    done, t0 = 0, time.perf_counter()
-    k = 0
+    rs = rng(0, n * NJ * 16)
+    tg = [actions_to_targets(defaults, rs[j * n * NJ:(j + 1) * n * NJ]).astype(np.float64).reshape(n, NJ)
+          for j in range(16)]
+    done, k = 0, 0
+    t0 = time.perf_counter()
     while True:
-        blk = rs[(k % 64) * n * NJ:((k % 64) + 1) * n * NJ]
-        t = defaults[None, :] + 0.6 * ((blk >> 8).astype(np.float64) * (1.0 / 8388608.0) - 1.0).reshape(n, NJ)
-        s.step(t, threads)
+        s.step(tg[k % 16], threads)
         done += n
         k += 1
         if time.perf_counter() - t0 >= seconds:
             break
     el = time.perf_counter() - t0
-    cpu = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(": \t") if "model name" in flags else "?"
+    cpu = flags.split("model name")[1].split("\n")[0].strip(": \t") if "model name" in flags else "?"
     return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"{n} worlds x {k} steps ({el:.1f} s), same seeds/difficulty/action law",
-            "build": kind, "cpu": cpu}
+            "sample": f"{n} worlds x {k} steps ({el:.1f} s), per-world difficulty U[0,1), same action law, "
+                      "arc_sim_step only", "build": kind, "cpu": cpu}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-worlds", type=int, default=4096)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--latency-steps", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["all", "kernel"], default="all")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested): worlds would shard by "
+              "HBM budget across GPUs with no exchange inside a step, and no multi-GPU run is part of it",
+              file=sys.stderr)
+        return 2
+    steps, warm = args.steps, args.warmup
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.only == "kernel":
+        # a plain 1 GiB device copy first: the counters' reading for known traffic
+        from arc import native
+        a, b = native.DeviceBuffer(1 << 30), native.DeviceBuffer(1 << 30)
+        native.sim_lib().arcx_memcpy(b.ptr, a.ptr, 1 << 30, 2)
+        a.close()
+        b.close()
+        kp = kernel_pass(steps, warm)
+        print(json.dumps({"kernel_pass": kp}), flush=True)
+        return 0
+
+    log("served form, 65536 worlds")
+    served = served_pass(steps, warm)
+    log(f"served {served['value']:.4g} world-steps/s (closed loop {served['closed_loop']['value']:.4g}); "
+        "batched graph form")
+    batched = batched_pass(steps, warm)
+    log(f"batched {batched['value']:.4g}; kernel pass")
+    kp = kernel_pass(steps, warm)
+    log(f"k_step {kp['kernel_ms_avg']:.3f} ms; plain form")
+    plain = plain_pass()
+    log(f"plain {plain['value']:.4g}; latency at 4096 worlds")
+    lat = latency_pass(args.latency_steps, warm)
+    log(f"latency p50 {lat['p50_ms']:.3f} p99 {lat['p99_ms']:.3f} ms; serving")
+    serving = serving_pass(args.serve_seconds)
+    log(f"serving p50 {serving['p50_ms']:.3f} p99 {serving['p99_ms']:.3f} ms")
+
+    avg_s = kp["kernel_ms_avg"] / 1e3
+    achieved = N * FLOP_PER_WORLD_STEP / avg_s
+    traffic = traffic_from_profile()
+    line = {
+        "metric": METRIC,
+        "value": served["value"],
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": served["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "65536 quadruped worlds served as 16 clients x 4096 (two step requests in flight "
+                               "per client), per-world difficulty U[0,1), uniform random actions, fall rules + "
+                               "auto-reset on",
+                   "worlds": N, "dt": 1.0 / 30.0, "substeps": 20, "grid": 64, "cell": 0.4,
+                   "form": "step server over the batched lane-form kernel (hipGraph)"},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": traffic,
+                     "kernel": "k_step", "kernel_ms_avg": kp["kernel_ms_avg"], "worlds_per_launch": N,
+                     "flop_per_world_step": FLOP_PER_WORLD_STEP,
+                     "hbm_frac": N * BYTES_PER_WORLD_STEP / avg_s / PEAK_HBM},
+        "forms": {"served": served, "batched_graph": batched, "plain": plain},
+        "latency_4096": lat,
+        "serving": serving,
+        "launch": kp["launch"],
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(args.cpu_worlds, args.cpu_seconds)
+        line["cpu_baseline"] = cpu
+        if cpu.get("value"):
+            line["ratios"] = {"served_vs_cpu": served["value"] / cpu["value"],
+                              "served_vs_plain": served["value"] / plain["value"],
+                              "batched_vs_plain": batched["value"] / plain["value"]}
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

@@ -106,8 +106,8 @@ int arcx_get_launch(const arc_sim *s, int *out4);
  * world-major (rows of worlds not in the batch are not read).  Ranges must be
  * inside [0, N) and disjoint (checked; -1 otherwise).  task = 1 adds the fall
  * rules and auto-reset of the stepped worlds (as arcx_step_task).  graph = 1
- * replays a captured hipGraph of the launch sequence; it is captured again
- * only when targets_dev, task or the launch configuration change, while the
+ * replays a captured hipGraph of the launch sequence: one graph per
+ * (targets_dev, task, launch configuration), the last four kept, while the
  * request table is read from device memory at every replay.  Asynchronous;
  * arcx_sync waits. */
 int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,

@@ -1,0 +1,95 @@
+"""GPU step server: admission by HBM budget, batched stepping of several
+clients' worlds, per-request latency statistics.  A client's worlds must
+evolve exactly as the same worlds in a sim of their own."""
+
+import threading
+
+import numpy as np
+import pytest
+
+from helpers import action_stream
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from arc import native, server
+    return native, server
+
+
+def test_admission_by_budget(mods):
+    native, server = mods
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(1000 * (per + 96))
+    assert 990 <= sv.capacity <= 1000
+    a = sv.open(600, seed0=0, difficulty=0.5)
+    with pytest.raises(server.AdmissionError):
+        sv.open(600)
+    b = sv.open(sv.capacity - 600, seed0=7)
+    assert sv.used == sv.capacity
+    sv.close(a)
+    assert sv.used == sv.capacity - 600
+    c = sv.open(500)
+    assert sv.range(c)[0] == 0 and sv.range(b)[0] == 600
+    sv.close_server()
+
+
+def test_clients_match_standalone_sims(mods):
+    native, server = mods
+    sizes, seeds, steps = [37, 100, 5], [0, 1000, 77], 3
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(400 * (per + 96))
+    cids = [sv.open(n, seed0=s, difficulty=0.5) for n, s in zip(sizes, seeds)]
+    refs = []
+    for n, s in zip(sizes, seeds):
+        r = native.Sim(n)
+        ids = np.arange(n, dtype=np.int32)
+        r.reset(ids, (ids + s).astype(np.uint32), np.full(n, 0.5))
+        refs.append(r)
+    acts = [action_stream(n, steps, seed=i + 1).astype(np.float32) for i, n in enumerate(sizes)]
+    for k in range(steps):
+        tickets = [sv.submit(c, acts[i][k]) for i, c in enumerate(cids)]
+        for t in tickets:
+            assert sv.wait(t) > 0.0
+        for i, r in enumerate(refs):
+            r.step(acts[i][k].astype(np.float64))
+    for i, c in enumerate(cids):
+        root, _, _, _ = refs[i].observe()
+        assert np.array_equal(sv.observe(c), root), f"client {i}"
+    st = sv.stats()
+    assert st["requests"] == steps * len(sizes) and st["world_steps"] == steps * sum(sizes)
+    assert st["batches"] >= steps and st["p99_ms"] >= st["p50_ms"] > 0.0
+    sv.close_server()
+
+
+def test_concurrent_clients(mods):
+    """Client threads submitting and waiting concurrently; every request is
+    stepped once and in order (the per-client step count is the time)."""
+    native, server = mods
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(2000 * (per + 96))
+    sizes = [1, 64, 300, 700, 9, 128]
+    cids = [sv.open(n, seed0=10 * i) for i, n in enumerate(sizes)]
+    steps = 6
+    errors = []
+
+    def run(i):
+        try:
+            t = action_stream(sizes[i], 1, seed=i)[0].astype(np.float32)
+            for _ in range(steps):
+                sv.wait(sv.submit(cids[i], t))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(sizes))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors
+    st = sv.stats()
+    assert st["requests"] == steps * len(sizes)
+    assert st["world_steps"] == steps * sum(sizes)
+    print(f"server: {st}")
+    sv.close_server()
