@@ -26,11 +26,11 @@ constexpr int GRAPHS = 4; /* captured graphs kept per handle */
 
 struct LaunchKey {
     const float *targets;
-    int task, block, tile, grid, persistent;
+    int task, block, tile, grid, persistent, leg;
     bool operator==(const LaunchKey &o) const
     {
         return targets == o.targets && task == o.task && block == o.block && tile == o.tile && grid == o.grid
-               && persistent == o.persistent;
+               && persistent == o.persistent && leg == o.leg;
     }
 };
 
@@ -102,16 +102,29 @@ static BatchState *batch_of(arc_sim *s)
     return b;
 }
 
-/* grid of a lane-form launch over `total` items (graph: the worst case N) */
+static int is_leg(const arc_sim *s) { return s->form == 2; }
+
+/* the leg form's per-lane contact lists, allocated on first use */
+hipError_t ensure_leg(arc_sim *s)
+{
+    if (s->dev.lcbuf)
+        return hipSuccess;
+    const Model &M = s->su.model;
+    const size_t slots = (size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]);
+    return hipMalloc((void **)&s->dev.lcbuf, slots * 6 * 4 * s->dev.stride * sizeof(float));
+}
+
+/* grid of a launch over `total` worlds (graph: the worst case N) */
 static int grid_for(const arc_sim *s, int total)
 {
+    const int items = (is_leg(s) ? 4 : 1) * total;
     if (s->persistent) {
-        int r = step_resident_blocks(s->block, s->tile);
-        return r > 0 ? r : (total + s->block - 1) / s->block;
+        int r = step_resident_blocks(s->block, s->tile, is_leg(s));
+        return r > 0 ? r : (items + s->block - 1) / s->block;
     }
     if (s->grid_blocks > 0)
         return s->grid_blocks;
-    return (total + s->block - 1) / s->block;
+    return (items + s->block - 1) / s->block;
 }
 
 /* the kernels of one batched control step, on the handle's stream */
@@ -121,7 +134,7 @@ static hipError_t enqueue(arc_sim *s, BatchState *b, const float *targets, int t
     if (s->persistent && (e = launch_fill_int(b->d_work, 0, 1, s->stream)) != hipSuccess)
         return e;
     if ((e = launch_step(s->dev, s->d_model, targets, b->d_batch, 0, s->block, s->tile, grid,
-                         s->persistent ? b->d_work : nullptr, s->stream)) != hipSuccess)
+                         s->persistent ? b->d_work : nullptr, is_leg(s), s->stream)) != hipSuccess)
         return e;
     if (!task)
         return hipSuccess;
@@ -169,20 +182,22 @@ static hipError_t graph_for(arc_sim *s, BatchState *b, const LaunchKey &k, hipGr
     return hipSuccess;
 }
 
-/* every world of the sim, lane form, in the configured distribution */
+/* every world of the sim, lane or leg form, in the configured distribution */
 hipError_t launch_lane_all(arc_sim *s, const float *targets)
 {
+    hipError_t e;
+    if (is_leg(s) && (e = ensure_leg(s)) != hipSuccess)
+        return e;
     if (!s->persistent)
         return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, s->grid_blocks, nullptr,
-                           s->stream);
+                           is_leg(s), s->stream);
     BatchState *b = batch_of(s);
     if (!b)
         return hipErrorOutOfMemory;
-    hipError_t e = launch_fill_int(b->d_work, 0, 1, s->stream);
-    if (e != hipSuccess)
+    if ((e = launch_fill_int(b->d_work, 0, 1, s->stream)) != hipSuccess)
         return e;
     return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, grid_for(s, s->n), b->d_work,
-                       s->stream);
+                       is_leg(s), s->stream);
 }
 
 extern "C" {
@@ -201,12 +216,13 @@ int arcx_set_persistent(arc_sim *s, int on)
     return 0;
 }
 
-int arcx_get_launch(const arc_sim *s, int *out4)
+int arcx_get_launch(const arc_sim *s, int *out5)
 {
-    out4[0] = s->block;
-    out4[1] = s->tile;
-    out4[2] = s->persistent ? step_resident_blocks(s->block, s->tile) : s->grid_blocks;
-    out4[3] = s->persistent;
+    out5[0] = s->block;
+    out5[1] = s->tile;
+    out5[2] = s->persistent ? step_resident_blocks(s->block, s->tile, is_leg(s)) : s->grid_blocks;
+    out5[3] = s->persistent;
+    out5[4] = s->form;
     return 0;
 }
 
@@ -234,10 +250,12 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
             set_error(s, -1, "arcx_step_batch: overlapping ranges", hipSuccess);
             return -1;
         }
-    if (s->form != 0) {
-        set_error(s, -1, "arcx_step_batch: the batched form is the lane form", hipSuccess);
+    if (s->form == 1) {
+        set_error(s, -1, "arcx_step_batch: the batched launch runs the lane or leg form", hipSuccess);
         return -1;
     }
+    if (is_leg(s))
+        HIPCHK_RET(s, ensure_leg(s), "arcx_step_batch: leg-form buffers", -1);
     BatchState *b = batch_of(s);
     if (!b)
         return -1;
@@ -271,7 +289,8 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
         HIPCHK_RET(s, enqueue(s, b, targets_dev, task, grid_for(s, std::max(total, 1))), "arcx_step_batch launch", -1);
         return 0;
     }
-    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, s->tile, grid_for(s, std::max(s->n, 1)), s->persistent };
+    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, s->tile, grid_for(s, std::max(s->n, 1)), s->persistent,
+                 is_leg(s) };
     hipGraphExec_t exec = nullptr;
     HIPCHK_RET(s, graph_for(s, b, k, &exec), "arcx_step_batch capture", -1);
     HIPCHK_RET(s, hipGraphLaunch(exec, s->stream), "arcx_step_batch graph", -1);
@@ -283,14 +302,14 @@ int arcx_graph_captures(const arc_sim *s) { return s->batch ? s->batch->captures
 /* Times every launch configuration of the lane form on this handle's worlds
  * and keeps the fastest.  The device state is copied aside first and put
  * back afterwards, so the worlds end where they started. */
-int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
+int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
 {
     if (s->n <= 0 || steps <= 0) {
         set_error(s, -1, "arcx_autotune: nothing to time", hipSuccess);
         return -1;
     }
-    if (s->form != 0) {
-        set_error(s, -1, "arcx_autotune: the tuner configures the lane form", hipSuccess);
+    if (s->form == 1) {
+        set_error(s, -1, "arcx_autotune: the tuner configures the lane and leg forms", hipSuccess);
         return -1;
     }
     const size_t nt = (size_t)s->n * NJ;
@@ -308,8 +327,9 @@ int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
     void *snap = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = -1;
-    const int keep_block = s->block, keep_tile = s->tile, keep_grid = s->grid_blocks, keep_pers = s->persistent;
-    int best[4] = { keep_block, keep_tile, keep_grid, keep_pers };
+    const int keep_block = s->block, keep_tile = s->tile, keep_grid = s->grid_blocks, keep_pers = s->persistent,
+              keep_form = s->form;
+    int best[5] = { keep_block, keep_tile, keep_grid, keep_pers, keep_form };
     double best_ms = 1e30;
     do {
         if (hipMalloc((void **)&dt, nt * sizeof(float)) != hipSuccess || hipMalloc(&snap, s->dev.bytes) != hipSuccess
@@ -320,10 +340,13 @@ int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
         bool ok = hipMemcpyAsync(dt, ht.data(), nt * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
                   && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
                   && hipStreamSynchronize(s->stream) == hipSuccess;
-        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 };
+        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[2] = { 0, 2 };
+        ok = ok && ensure_leg(s) == hipSuccess;
+        for (int fi = 0; fi < 2 && ok; ++fi)
         for (int bi = 0; bi < 3 && ok; ++bi)
             for (int ti = 0; ti < 2 && ok; ++ti)
                 for (int pi = 0; pi < 2 && ok; ++pi) {
+                    s->form = forms[fi];
                     s->block = blocks[bi];
                     s->tile = tiles[ti];
                     s->grid_blocks = 0;
@@ -347,6 +370,7 @@ int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
                         best[1] = s->tile;
                         best[2] = 0;
                         best[3] = s->persistent;
+                        best[4] = s->form;
                     }
                 }
         if (!ok) {
@@ -363,7 +387,8 @@ int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
         s->tile = best[1];
         s->grid_blocks = best[2];
         s->persistent = best[3];
-        arcx_get_launch(s, out4);
+        s->form = best[4];
+        arcx_get_launch(s, out5);
         if (ms_out)
             *ms_out = best_ms;
     } else {
@@ -371,6 +396,7 @@ int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out)
         s->tile = keep_tile;
         s->grid_blocks = keep_grid;
         s->persistent = keep_pers;
+        s->form = keep_form;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

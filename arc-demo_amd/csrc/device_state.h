@@ -46,6 +46,7 @@ struct DevWorlds {
     float *outside;   /* [stride] height outside the grid */
     float *difficulty; /* [stride] terrain difficulty of the last reset */
     float *cbuf;      /* [cslots*6][stride] per-substep contact list */
+    float *lcbuf;     /* leg form's contact lists, [slot*6+field][4*stride] (allocated on first use) */
     int *err;         /* [stride] contact-list overflow bits (sticky) */
 };
 
@@ -126,22 +127,23 @@ __host__ __device__ inline TerrainRef terrain_ref(const DevWorlds &d, size_t i)
 void dev_free(DevWorlds &d);
 hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, float outside, hipStream_t s);
 /* targets: n*NJ world-major, double (host ABI) or float (device inputs) */
-/* step kernel, lane form.  batch (device) or worlds 0..total-1; tile 0 or 16
- * (LDS terrain tile side); grid_blocks 0 = one work item per thread; work
- * (device counter, zeroed beforehand) selects the persistent distribution. */
+/* step kernel.  batch (device) or worlds 0..total-1; tile 0 or 16 (LDS
+ * terrain tile side); grid_blocks 0 = one work item per thread; work (device
+ * counter, zeroed beforehand) selects the persistent distribution; leg = 1:
+ * four lanes per world (leg_engine.h; needs d.lcbuf), else one. */
 hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch, int total,
-                       int block, int tile, int grid_blocks, int *work, hipStream_t s);
+                       int block, int tile, int grid_blocks, int *work, int leg, hipStream_t s);
 hipError_t launch_step_b64(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
-                           int total, int tile, int grid_blocks, int *work, hipStream_t s);
+                           int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s);
 hipError_t launch_step_b128(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
-                            int total, int tile, int grid_blocks, int *work, hipStream_t s);
+                            int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s);
 hipError_t launch_step_b256(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
-                            int total, int tile, int grid_blocks, int *work, hipStream_t s);
+                            int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s);
 /* workgroups of the step kernel (block, tile) resident on the device at once */
-int step_resident_blocks(int block, int tile);
-int step_resident_blocks_b64(int tile);
-int step_resident_blocks_b128(int tile);
-int step_resident_blocks_b256(int tile);
+int step_resident_blocks(int block, int tile, int leg);
+int step_resident_blocks_b64(int tile, int leg);
+int step_resident_blocks_b128(int tile, int leg);
+int step_resident_blocks_b256(int tile, int leg);
 /* Resets ids[k] with seeds[k] and difficulty (double or float array), for k <
  * *count_dev when count_dev is given (entries decided on the device), else
  * k < n_ids; at most `max_blocks` blocks loop over the entries. */

@@ -41,6 +41,7 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
         return e;
     d.base = p;
     d.bytes = bytes;
+    d.lcbuf = nullptr;
     float *f = (float *)p;
     d.x = f; f += NB * 3 * S;
     d.q = f; f += NB * 4 * S;
@@ -75,24 +76,24 @@ void dev_free(DevWorlds &d)
 
 
 hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch, int total,
-                       int block, int tile, int grid_blocks, int *work, hipStream_t s)
+                       int block, int tile, int grid_blocks, int *work, int leg, hipStream_t s)
 {
     if (!batch && total <= 0)
         return hipSuccess;
     switch (block) {
-    case 64: return launch_step_b64(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
-    case 128: return launch_step_b128(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
-    case 256: return launch_step_b256(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+    case 64: return launch_step_b64(d, m_dev, targets, batch, total, tile, grid_blocks, work, leg, s);
+    case 128: return launch_step_b128(d, m_dev, targets, batch, total, tile, grid_blocks, work, leg, s);
+    case 256: return launch_step_b256(d, m_dev, targets, batch, total, tile, grid_blocks, work, leg, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-int step_resident_blocks(int block, int tile)
+int step_resident_blocks(int block, int tile, int leg)
 {
     switch (block) {
-    case 64: return step_resident_blocks_b64(tile);
-    case 128: return step_resident_blocks_b128(tile);
-    case 256: return step_resident_blocks_b256(tile);
+    case 64: return step_resident_blocks_b64(tile, leg);
+    case 128: return step_resident_blocks_b128(tile, leg);
+    case 256: return step_resident_blocks_b256(tile, leg);
     default: return 0;
     }
 }

@@ -183,6 +183,8 @@ void arc_sim_destroy(arc_sim *s)
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
     batch_free(s);
+    if (s->dev.lcbuf)
+        (void)hipFree(s->dev.lcbuf);
     dev_free(s->dev);
     if (s->d_model) (void)hipFree(s->d_model);
     if (s->d_reset) (void)hipFree(s->d_reset);
@@ -557,8 +559,10 @@ int arcx_sync(arc_sim *s)
 
 int arcx_set_form(arc_sim *s, int form)
 {
-    if (form < 0 || form > 1)
+    if (form < 0 || form > 2)
         return -1;
+    if (form == 2)
+        HIPCHK_RET(s, ensure_leg(s), "arcx_set_form: leg-form buffers", -1);
     s->form = form;
     return 0;
 }

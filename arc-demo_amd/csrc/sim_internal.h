@@ -38,7 +38,7 @@ struct arc_sim {
     int block, grid_blocks;
     int tile;              /* LDS terrain tile side of the lane form: 0 or 16 */
     int persistent;        /* lane form: wavefronts take work from a counter */
-    int form;              /* 0: one world per lane, 1: one world per wavefront */
+    int form;              /* 0: one world per lane, 1: one world per wavefront, 2: four lanes per world */
     WaveK wave;
     WaveK *d_wave;
     TaskK task;
@@ -56,8 +56,10 @@ void set_error(const arc_sim *cs, int code, const char *what, hipError_t e);
 hipError_t launch_form(arc_sim *s, const float *targets);
 /* releases the batch state of a handle */
 void batch_free(arc_sim *s);
-/* every world of the sim, lane form, in the configured distribution */
+/* every world of the sim, lane or leg form, in the configured distribution */
 hipError_t launch_lane_all(arc_sim *s, const float *targets);
+/* the leg form's contact lists (allocated on first use) */
+hipError_t ensure_leg(arc_sim *s);
 
 #define HIPCHK(s, call, what)                                 \
     do {                                                      \

@@ -4,11 +4,11 @@
 namespace xpbd {
 
 hipError_t launch_step_b128(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
-                          int total, int tile, int grid_blocks, int *work, hipStream_t s)
+                          int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s)
 {
-    return launch_step_block<128>(d, m_dev, targets, batch, total, tile, grid_blocks, work, s);
+    return launch_step_block<128>(d, m_dev, targets, batch, total, tile, grid_blocks, work, leg, s);
 }
 
-int step_resident_blocks_b128(int tile) { return step_resident_blocks_of<128>(tile); }
+int step_resident_blocks_b128(int tile, int leg) { return step_resident_blocks_of<128>(tile, leg); }
 
 } /* namespace xpbd */

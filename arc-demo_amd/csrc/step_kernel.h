@@ -16,6 +16,7 @@
  */
 #pragma once
 #include "device_state.h"
+#include "leg_engine.h"
 
 namespace xpbd {
 
@@ -130,13 +131,13 @@ template <int BLOCK, int TILE> struct LdsTile {
     }
 };
 
-/* fill this thread's tile of world i around the torso position (x, y) */
-template <int BLOCK, int TILE>
-__device__ __forceinline__ LdsTile<BLOCK, TILE> fill_tile(int8_t *s_tile, const DevWorlds &d, const TerrainK &t,
-                                                          size_t i, float x, float y)
+/* fill the tile of world i around the torso position (x, y); lt = the
+ * world's column 0 in LDS, columns STRIDE bytes apart */
+template <int STRIDE, int TILE>
+__device__ __forceinline__ LdsTile<STRIDE, TILE> fill_tile(int8_t *lt, const DevWorlds &d, const TerrainK &t,
+                                                           size_t i, float x, float y)
 {
-    LdsTile<BLOCK, TILE> T;
-    int8_t *lt = s_tile + threadIdx.x;
+    LdsTile<STRIDE, TILE> T;
     T.lt = lt;
     const int n = d.tpitch - 2 * TPAD;
     if (d.tpitch < TILE) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ LdsTile<BLOCK, TILE> fill_tile(int8_t *s_tile, const 
             __builtin_memcpy(&word, src + v * d.tpitch + 4 * w, 4);
 #pragma unroll
             for (int b = 0; b < 4; ++b)
-                lt[(4 * w + b + v * TILE) * BLOCK] = (int8_t)(word >> (8 * b));
+                lt[(4 * w + b + v * TILE) * STRIDE] = (int8_t)(word >> (8 * b));
         }
     }
     return T;
@@ -199,7 +200,7 @@ __device__ __forceinline__ void step_world(const DevWorlds &d, const Model &M, c
     int ov;
     if constexpr (TILE > 0) {
         const LdsTile<BLOCK, TILE> tile =
-            fill_tile<BLOCK, TILE>(s_tile, d, M.terr, (size_t)i, d.x[0 * S + i], d.x[1 * S + i]);
+            fill_tile<BLOCK, TILE>(s_tile + threadIdx.x, d, M.terr, (size_t)i, d.x[0 * S + i], d.x[1 * S + i]);
         load_world(d, i, W);
         ov = lane_step(W, A, P0, M, tr, cs, tile);
     } else {
@@ -246,16 +247,151 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* leg form: the four lanes of a quad step one world (leg_engine.h)     */
+
+__device__ __forceinline__ void load_leg(const DevWorlds &d, size_t i, int q, LegWorld &W)
+{
+    const size_t S = d.stride;
+    auto body = [&](int b, Body &B) {
+        B.x = v3(d.x[(b * 3 + 0) * S + i], d.x[(b * 3 + 1) * S + i], d.x[(b * 3 + 2) * S + i]);
+        B.q = q4(d.q[(b * 4 + 0) * S + i], d.q[(b * 4 + 1) * S + i], d.q[(b * 4 + 2) * S + i], d.q[(b * 4 + 3) * S + i]);
+        B.v = v3(d.v[(b * 3 + 0) * S + i], d.v[(b * 3 + 1) * S + i], d.v[(b * 3 + 2) * S + i]);
+        B.w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
+    };
+    body(0, W.T);
+    body(1 + 3 * q, W.b[0]);
+    body(2 + 3 * q, W.b[1]);
+    body(3 + 3 * q, W.b[2]);
+    const uint32_t a = d.anchored[i];
+    W.anchored = (a & 1u) | (((a >> (5 * q + 1)) & 31u) << 1);
+    W.unstable = d.unstable[i];
+    W.time = d.time[i];
+}
+
+__device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, const LegWorld &W, int ov)
+{
+    const size_t S = d.stride;
+    auto body = [&](int b, const Body &B) {
+        d.x[(b * 3 + 0) * S + i] = B.x.x; d.x[(b * 3 + 1) * S + i] = B.x.y; d.x[(b * 3 + 2) * S + i] = B.x.z;
+        d.q[(b * 4 + 0) * S + i] = B.q.w; d.q[(b * 4 + 1) * S + i] = B.q.x;
+        d.q[(b * 4 + 2) * S + i] = B.q.y; d.q[(b * 4 + 3) * S + i] = B.q.z;
+        d.v[(b * 3 + 0) * S + i] = B.v.x; d.v[(b * 3 + 1) * S + i] = B.v.y; d.v[(b * 3 + 2) * S + i] = B.v.z;
+        d.w[(b * 3 + 0) * S + i] = B.w.x; d.w[(b * 3 + 1) * S + i] = B.w.y; d.w[(b * 3 + 2) * S + i] = B.w.z;
+    };
+    body(1 + 3 * q, W.b[0]);
+    body(2 + 3 * q, W.b[1]);
+    body(3 + 3 * q, W.b[2]);
+    for (int m = 0; m < 3; ++m) {
+        const int j = 3 * q + m;
+        d.jout[(0 * NJ + j) * S + i] = W.angle[m];
+        d.jout[(1 * NJ + j) * S + i] = W.rate[m];
+        d.jout[(2 * NJ + j) * S + i] = W.torque[m];
+    }
+    for (int k = 1; k < 6; ++k)
+        d.touch[(5 * q + k) * S + i] = (uint8_t)leg_touch(W, k);
+    const int anch = qor((int)(((W.anchored >> 1) & 31u) << (5 * q + 1)));
+    const int err = qor(ov);
+    if (q == 0) {
+        body(0, W.T);
+        d.touch[i] = (uint8_t)leg_touch(W, 0);
+        d.anchored[i] = (uint32_t)anch | (W.anchored & 1u);
+        d.unstable[i] = W.unstable;
+        d.time[i] = W.time;
+        if (err)
+            d.err[i] |= err;
+    }
+}
+
+/* one control step of world i by lane q of its quad */
+template <int BLOCK, int TILE>
+__device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+                                               LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q)
+{
+    const size_t S = d.stride;
+    LegWorld W;
+    for (int m = 0; m < 3; ++m) {
+        const int j = 3 * q + m;
+        float tg = targets[(size_t)i * NJ + j];
+        W.target[m] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
+        d.target[j * S + i] = W.target[m];
+    }
+    if (d.unstable[i])
+        return; /* frozen until reset, csrc/arc_sim.c:135-136 */
+    GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
+    tr.outside = d.outside[i];
+    ContactStore cs;
+    cs.base = d.lcbuf;
+    cs.stride = 4 * S;
+    cs.world = (size_t)i * 4 + q;
+    constexpr int WPB = BLOCK / 4;
+    int ov;
+    if constexpr (TILE > 0) {
+        /* the quad's four lanes write the same bytes: no ordering needed */
+        const LdsTile<WPB, TILE> tile = fill_tile<WPB, TILE>(s_tile + (threadIdx.x >> 2), d, M.terr, (size_t)i,
+                                                             d.x[0 * S + i], d.x[1 * S + i]);
+        load_leg(d, i, q, W);
+        ov = leg_step(W, q, A, P0, M, tr, cs, tile);
+    } else {
+        load_leg(d, i, q, W);
+        ov = leg_step(W, q, A, P0, M, tr, cs, GridTile());
+    }
+    store_leg(d, opaque_index((size_t)i), q, W, ov);
+}
+
+template <int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void k_step_leg(DevWorlds d, const Model *__restrict__ m,
+                                                    const float *__restrict__ targets,
+                                                    const StepBatch *__restrict__ batch, int total,
+                                                    int *__restrict__ work)
+{
+    __shared__ float s_pose[4 * 7 * BLOCK];
+    __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * (BLOCK / 4) : 1];
+    const Model &M = *m;
+    LdsPoses<BLOCK> P0;
+    P0.p = s_pose;
+    P0.tid = threadIdx.x;
+    const int n_ranges = batch ? batch->n_ranges : 0;
+    if (batch)
+        total = batch->total;
+    const int items = 4 * total;
+    const int lane = threadIdx.x & 63;
+    auto next = [&](int base) {
+        if (!work)
+            return base + (int)gridDim.x * BLOCK;
+        int v = 0;
+        if (lane == 0)
+            v = atomicAdd(work, 64);
+        return __shfl(v, 0, 64);
+    };
+    int base = work ? next(0) : (int)blockIdx.x * BLOCK + (threadIdx.x & ~63);
+    while (base < items) {
+        const int t = base + lane;
+        if (t < items) {
+            const int w = t >> 2;
+            step_world_leg<BLOCK, TILE>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, w) : w, t & 3);
+        }
+        base = next(base);
+    }
+}
+
 /* workgroups of the BLOCK-thread kernel resident on the whole device at once */
 template <int BLOCK>
-int step_resident_blocks_of(int tile)
+int step_resident_blocks_of(int tile, int leg)
 {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess
         || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    hipError_t e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 16>, BLOCK, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 0>, BLOCK, 0);
+    hipError_t e;
+    if (leg)
+        e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16>, BLOCK, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 0>, BLOCK, 0);
+    else
+        e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 16>, BLOCK, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 0>, BLOCK, 0);
     if (e != hipSuccess)
         return 0;
     return per * cus;
@@ -264,12 +400,25 @@ int step_resident_blocks_of(int tile)
 /* launcher of the BLOCK-thread kernels (grid_blocks 0: one item per thread) */
 template <int BLOCK>
 hipError_t launch_step_block(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
-                             int total, int tile, int grid_blocks, int *work, hipStream_t s)
+                             int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s)
 {
     if (grid_blocks <= 0)
-        grid_blocks = (total + BLOCK - 1) / BLOCK;
+        grid_blocks = ((leg ? 4 : 1) * total + BLOCK - 1) / BLOCK;
     if (grid_blocks <= 0)
         return hipSuccess;
+    if (leg) {
+        if (!d.lcbuf)
+            return hipErrorInvalidValue;
+        if (tile == 16)
+            hipLaunchKernelGGL((k_step_leg<BLOCK, 16>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch,
+                               total, work);
+        else if (tile == 0)
+            hipLaunchKernelGGL((k_step_leg<BLOCK, 0>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch,
+                               total, work);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (tile == 16)
         hipLaunchKernelGGL((k_step<BLOCK, 16>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch, total,
                            work);

@@ -74,7 +74,8 @@ int arcx_task_episodes(const arc_sim *s, unsigned int *out);
 
 /* Form of the step kernel: 0 = one world per lane (default; all substeps
  * of a control step in one launch, world state in registers), 1 = the plain
- * form, one world per wavefront with the world staged in LDS. */
+ * form, one world per wavefront with the world staged in LDS, 2 = four lanes
+ * per world (one leg each, torso replicated; shorter per-world latency). */
 int arcx_set_form(arc_sim *s, int form);
 
 /* Launch shape of the step kernel: threads per block (a multiple of 64) and
@@ -87,7 +88,7 @@ double arcx_bytes_per_world(const arc_sim *s);
 /* Contact-list overflow flags of world i (0 expected). */
 int arcx_world_error(const arc_sim *s, int i);
 
-/* ---- batched, persistent and graph form (lane form only) -------------
+/* ---- batched, persistent and graph form (lane and leg forms) ---------
  * No counterpart in the reference, whose arc_sim_step always steps every
  * world (csrc/arc_sim.c:123-138); these serve callers that step several
  * independent requests' worlds together. */
@@ -98,8 +99,9 @@ int arcx_set_layout(arc_sim *s, int tile);
 /* 1: persistent distribution, a grid of resident workgroups whose
  * wavefronts take 64 worlds at a time from a device counter. */
 int arcx_set_persistent(arc_sim *s, int on);
-/* out4 = threads per block, tile, blocks (persistent: resident blocks), persistent */
-int arcx_get_launch(const arc_sim *s, int *out4);
+/* out5 = threads per block, tile, blocks (persistent: resident blocks),
+ * persistent, form */
+int arcx_get_launch(const arc_sim *s, int *out5);
 
 /* One control step of the worlds [starts[r], starts[r] + counts[r]) of n_req
  * (<= 256) requests in one launch.  targets_dev: device float N x 12,
@@ -115,11 +117,11 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
 /* graphs captured so far by arcx_step_batch */
 int arcx_graph_captures(const arc_sim *s);
 
-/* Autotuner: times every (threads per block, LDS layout, distribution) of
- * the lane form over `steps` control steps of this handle's own worlds from
- * their current state (restored afterwards), keeps the fastest and reports it
- * in out4 (as arcx_get_launch) and its ms per step in ms_out. */
-int arcx_autotune(arc_sim *s, int steps, int *out4, double *ms_out);
+/* Autotuner: times every (form: lane or leg, threads per block, LDS
+ * layout, distribution) over `steps` control steps of this handle's own
+ * worlds from their current state (restored afterwards), keeps the fastest
+ * and reports it in out5 (as arcx_get_launch) and its ms per step in ms_out. */
+int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out);
 
 #ifdef __cplusplus
 }
