@@ -39,6 +39,7 @@ def server_lib() -> ctypes.CDLL:
         ("arcs_stats", _int, [_vp, _vp]),
         ("arcs_reset_stats", None, [_vp]),
         ("arcs_set_window", _int, [_vp, _dbl]),
+        ("arcs_autotune", _int, [_vp, _int, _vp, ctypes.POINTER(_dbl)]),
         ("arcs_observe", _int, [_vp, _int, _vp]),
     ]:
         fn = getattr(lib, name)
@@ -111,6 +112,14 @@ class StepServer:
 
     def reset_stats(self) -> None:
         self._lib.arcs_reset_stats(self._ptr)
+
+    def autotune(self, steps: int = 3) -> tuple[dict, float]:
+        from .native import _launch_dict
+        out = np.zeros(5, dtype=np.int32)
+        ms = _dbl(0.0)
+        if self._lib.arcs_autotune(self._ptr, steps, out.ctypes.data, ctypes.byref(ms)):
+            raise RuntimeError("arcs_autotune failed")
+        return _launch_dict(out), ms.value
 
     def set_window(self, microseconds: float) -> None:
         if self._lib.arcs_set_window(self._ptr, microseconds):

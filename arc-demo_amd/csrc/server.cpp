@@ -471,6 +471,19 @@ int arcs_stats(const arc_server *cs, double *out)
     return 0;
 }
 
+int arcs_autotune(arc_server *sv, int steps, int *out5, double *ms)
+{
+    /* no batch is launched meanwhile; the tuner's copies follow the batches
+     * already on the stream, and it restores the worlds it times */
+    std::lock_guard<std::mutex> sl(sv->sim_mu);
+    int rc = arcx_autotune(sv->sim, steps, out5, ms);
+    if (rc) {
+        fprintf(stderr, "libarcsim: arcs_autotune: %s\n", arcx_error_message(sv->sim));
+        arcx_clear_error(sv->sim);
+    }
+    return rc;
+}
+
 int arcs_set_window(arc_server *sv, double microseconds)
 {
     if (!(microseconds >= 0.0))

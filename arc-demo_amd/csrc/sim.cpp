@@ -576,7 +576,13 @@ int arcx_set_launch(arc_sim *s, int block, int grid_blocks)
     return 0;
 }
 
-double arcx_bytes_per_world(const arc_sim *s) { return (double)dev_bytes_per_world(s->su.grid, s->su.model.cslots); }
+double arcx_bytes_per_world(const arc_sim *s)
+{
+    /* the world arrays plus the leg form's contact lists (held once that form is used) */
+    const Model &M = s->su.model;
+    const double leg = (double)(M.ccap[0] + M.ccap[1] + M.ccap[5]) * 6 * 4 * sizeof(float);
+    return (double)dev_bytes_per_world(s->su.grid, M.cslots) + leg;
+}
 
 int arcx_world_error(const arc_sim *cs, int i)
 {

@@ -342,7 +342,7 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
 }
 
 template <int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) void k_step_leg(DevWorlds d, const Model *__restrict__ m,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_step_leg(DevWorlds d, const Model *__restrict__ m,
                                                     const float *__restrict__ targets,
                                                     const StepBatch *__restrict__ batch, int total,
                                                     int *__restrict__ work)

@@ -54,7 +54,7 @@ PEAK_HBM = 8.0e12                # MI355X HBM3E, B/s
 NJ = 12
 N = 65536
 CLIENTS = 16
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_step_lane_65536w_pmc.md"
+PROFILES = ROOT / "profiles"
 
 
 def rng(seed: int, n: int) -> np.ndarray:
@@ -80,16 +80,24 @@ def new_sim(n: int, difficulty=None):
     return s
 
 
+def warm_and_tune(s, n, t, warm):
+    """--warmup steps with resets (the contact load settles after ~80 steps),
+    then the autotuner on the settled worlds"""
+    per = n * NJ
+    for k in range(warm):
+        s.step_batch([0], [n], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+    s.sync()
+    cfg, _ = s.autotune(3)
+    return cfg
+
+
 def kernel_pass(steps: int, warm: int) -> dict:
-    """k_step at 65,536 worlds: HIP events around each launch on the engine stream."""
+    """the step kernel at 65,536 worlds: HIP events around each launch on the engine stream"""
     from arc import native
     s = new_sim(N)
-    cfg, _ = s.autotune(3)
     t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
     per = N * NJ
-    for k in range(warm):
-        s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
-    s.sync()
+    cfg = warm_and_tune(s, N, t, warm)
     kms = 0.0
     for k in range(steps):
         m, _ = s.time_steps(t.ptr + 4 * per * (k % 8), 1, per)
@@ -101,12 +109,9 @@ def kernel_pass(steps: int, warm: int) -> dict:
 def batched_pass(steps: int, warm: int) -> dict:
     from arc import native
     s = new_sim(N)
-    cfg, _ = s.autotune(3)
     t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
     per = N * NJ
-    for k in range(warm):
-        s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
-    s.sync()
+    cfg = warm_and_tune(s, N, t, warm)
     t0 = time.perf_counter()
     for k in range(steps):
         s.step_batch([0], [N], t.ptr + 4 * per * ((warm + k) % 8), task=True, graph=True)
@@ -138,16 +143,15 @@ def latency_pass(steps: int, warm: int) -> dict:
     from arc import native
     n = 4096
     s = new_sim(n, 0.5)
-    cfg, _ = s.autotune(3)
     t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, n * NJ * 8)))
     per = n * NJ
+    cfg = warm_and_tune(s, n, t, warm)
     lat = []
-    for k in range(warm + steps):
+    for k in range(steps):
         t0 = time.perf_counter()
         s.step_batch([0], [n], t.ptr + 4 * per * (k % 8), task=True, graph=True)
         s.sync()
-        if k >= warm:
-            lat.append(1e3 * (time.perf_counter() - t0))
+        lat.append(1e3 * (time.perf_counter() - t0))
     lat = np.array(lat)
     return {"worlds": n, "difficulty": 0.5, "steps": steps, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
@@ -178,7 +182,9 @@ def served_pass(steps: int, warm: int) -> dict:
                 sv.wait(x)
 
     out = {"clients": CLIENTS, "worlds_per_client": n}
-    k = 0
+    run(0, warm, 1)
+    out["launch"], _ = sv.autotune(3)
+    k = warm
     for depth, name in ((1, "closed_loop"), (2, "pipelined")):
         run(k, warm, depth)
         k += warm
@@ -207,6 +213,10 @@ def serving_pass(seconds: float) -> dict:
     cids = [sv.open(n, seed0=10000 * c, difficulty=-1.0) for c, n in enumerate(sizes)]
     defaults = native.Sim(1).default_angles
     acts = [actions_to_targets(defaults, rng(2000 + c, n * NJ)) for c, n in enumerate(sizes)]
+    for _ in range(30):  # settle the worlds, then tune the launch on them
+        for x in [sv.submit(c, acts[c]) for c in cids]:
+            sv.wait(x)
+    launch, _ = sv.autotune(3)
     # saturation: every client keeps one request in flight
     stop = time.perf_counter() + min(1.0, seconds / 3)
     for c in cids:
@@ -242,18 +252,24 @@ def serving_pass(seconds: float) -> dict:
         sv.wait(x)
     st = sv.stats()
     sv.close_server()
-    return {"clients": CLIENTS, "worlds": int(sum(sizes)), "sizes": sizes,
+    return {"clients": CLIENTS, "worlds": int(sum(sizes)), "sizes": sizes, "launch": launch,
             "saturation_requests_per_s": sat["requests"] / sat["seconds"],
             "saturation_world_steps_per_s": sat["world_steps_per_s"], "offered_requests_per_s": rate,
             "requests": int(st["requests"]), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
             "world_steps_per_s": st["world_steps_per_s"], "requests_per_batch": st["requests_per_batch"]}
 
 
-def traffic_from_profile():
-    """HBM bytes per k_step launch from the committed counter summary (or None)."""
-    if not TRAFFIC_PROFILE.exists():
+def kernel_name(launch: dict) -> str:
+    return "k_step_leg" if launch.get("form") == "leg" else "k_step"
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` at 65,536 worlds from the committed
+    counter summary (FETCH_SIZE / WRITE_SIZE, calibrated), or None."""
+    f = PROFILES / f"{kernel}_65536w_pmc.md"
+    if not f.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_PROFILE.read_text())
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -296,7 +312,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-worlds", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -332,7 +348,7 @@ def main() -> int:
     batched = batched_pass(steps, warm)
     log(f"batched {batched['value']:.4g}; kernel pass")
     kp = kernel_pass(steps, warm)
-    log(f"k_step {kp['kernel_ms_avg']:.3f} ms; plain form")
+    log(f"{kernel_name(kp['launch'])} {kp['kernel_ms_avg']:.3f} ms; plain form")
     plain = plain_pass()
     log(f"plain {plain['value']:.4g}; latency at 4096 worlds")
     lat = latency_pass(args.latency_steps, warm)
@@ -342,7 +358,8 @@ def main() -> int:
 
     avg_s = kp["kernel_ms_avg"] / 1e3
     achieved = N * FLOP_PER_WORLD_STEP / avg_s
-    traffic = traffic_from_profile()
+    kname = kernel_name(kp["launch"])
+    traffic = traffic_from_profile(kname)
     line = {
         "metric": METRIC,
         "value": served["value"],
@@ -363,7 +380,7 @@ def main() -> int:
                    "form": "step server over the batched lane-form kernel (hipGraph)"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": traffic,
-                     "kernel": "k_step", "kernel_ms_avg": kp["kernel_ms_avg"], "worlds_per_launch": N,
+                     "kernel": kname, "kernel_ms_avg": kp["kernel_ms_avg"], "worlds_per_launch": N,
                      "flop_per_world_step": FLOP_PER_WORLD_STEP,
                      "hbm_frac": N * BYTES_PER_WORLD_STEP / avg_s / PEAK_HBM},
         "forms": {"served": served, "batched_graph": batched, "plain": plain},

@@ -82,7 +82,7 @@ int arcx_set_form(arc_sim *s, int form);
  * blocks (0 = one pass over the worlds). */
 int arcx_set_launch(arc_sim *s, int block, int grid_blocks);
 
-/* HBM bytes the library holds per world. */
+/* HBM bytes the library holds per world (every form's buffers). */
 double arcx_bytes_per_world(const arc_sim *s);
 
 /* Contact-list overflow flags of world i (0 expected). */

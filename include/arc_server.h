@@ -53,6 +53,10 @@ long long arcs_submit(arc_server *sv, int client, const float *targets);
  * milliseconds (submit to completion), or -1 for an unknown ticket. */
 double arcs_wait(arc_server *sv, long long ticket);
 
+/* Runs the sim's autotuner (arcx_autotune) on the worlds held now, between
+ * batches; out5 / ms as arcx_autotune.  Call it once the clients are open. */
+int arcs_autotune(arc_server *sv, int steps, int *out5, double *ms);
+
 /* Batching window: a batch waits up to this long after its oldest request
  * for requests of the other open clients, and starts at once when every
  * open client has one pending (default 500 us; 0 = never wait). */
