@@ -110,6 +110,10 @@ hipError_t ensure_leg(arc_sim *s)
     if (s->dev.lcbuf)
         return hipSuccess;
     const Model &M = s->su.model;
+    /* the form records contacts per lane without the per-world running
+     * total: it needs every sphere window to fit the 128-contact cap */
+    if (M.cslots > MAXC)
+        return hipErrorNotSupported;
     const size_t slots = (size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]);
     return hipMalloc((void **)&s->dev.lcbuf, slots * 6 * 4 * s->dev.stride * sizeof(float));
 }
@@ -341,9 +345,9 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
                   && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
                   && hipStreamSynchronize(s->stream) == hipSuccess;
         const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[2] = { 0, 2 };
-        ok = ok && ensure_leg(s) == hipSuccess;
+        const bool leg_ok = ensure_leg(s) == hipSuccess; /* else lane form only */
         for (int fi = 0; fi < 2 && ok; ++fi)
-        for (int bi = 0; bi < 3 && ok; ++bi)
+        for (int bi = 0; bi < 3 && ok && (forms[fi] != 2 || leg_ok); ++bi)
             for (int ti = 0; ti < 2 && ok; ++ti)
                 for (int pi = 0; pi < 2 && ok; ++pi) {
                     s->form = forms[fi];

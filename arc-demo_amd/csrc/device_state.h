@@ -13,6 +13,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include "lane_engine.h"
+#include "reset_engine.h"
 
 namespace xpbd {
 
@@ -56,6 +57,8 @@ struct ResetK {
     float stand, footprint;
     int grid;         /* generated grid size n (n x n columns) */
     float cell;
+    int chunk;        /* draws per lane of a generated reset (64 lanes) */
+    RngMatrix jump[JUMP_BITS]; /* chunk * 2^b draws ahead */
 };
 
 /* Template state broadcast at creation (the robot as built, flat ground). */

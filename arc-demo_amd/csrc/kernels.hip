@@ -122,8 +122,11 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
     const size_t S = d.stride;
     const int pitch = d.tpitch;
 
-    /* the draws are one sequential stream: lane 0 produces them */
-    if (tid == 0)
+    /* the draws: one stream, cut into 64 contiguous chunks that the lanes
+     * reach by jumping ahead (without terrain: 13 draws, lane 0) */
+    if (gen)
+        reset_draws_lane(seeds[k], gen, n, R.jump, R.chunk, tid, lat, hbuf, draws);
+    else if (tid == 0)
         reset_draws(seeds[k], gen, n, lat, hbuf, draws);
     __syncthreads();
     /* quantisation steps of the columns (csrc/arc_terrain.c:54-71) */
@@ -150,11 +153,14 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
     tl.base = base;
     tl.outside = -100.0f;
 
+    /* the footprint scan split over the lanes, then the maximum */
+    float ground = ground_part(M.terr, tl, R.footprint, R.cell, tid, 64);
+    for (int off = 32; off; off >>= 1)
+        ground = maxf_(ground, __shfl_xor(ground, off, 64));
     if (tid == 0) {
         float ang[NJ];
         for (int j = 0; j < NJ; ++j)
             ang[j] = clampf_(R.def[j] + draws[j], R.lo[j], R.hi[j]);
-        float ground = ground_under(M.terr, tl, R.footprint, R.cell);
         V3 bx[NB];
         Q4 bq[NB];
         pose_bodies(M, v3(0.0f, 0.0f, ground + R.stand + 0.03f), qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]), ang, bx, bq);

@@ -24,6 +24,33 @@
 
 namespace xpbd {
 
+/* The form relies on the legs differing only in joint geometry: every leg's
+ * bodies, spheres and joint gains / limits equal leg 0's (checked here on
+ * the generated constants). */
+constexpr bool same_v3(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+constexpr bool legs_alike()
+{
+    for (int l = 1; l < LEGS; ++l) {
+        for (int m = 0; m < 3; ++m) {
+            const BodyK &a = robot::body[1 + m], &b = robot::body[1 + 3 * l + m];
+            if (!(a.mass == b.mass && a.invm == b.invm && same_v3(a.I, b.I) && same_v3(a.invI, b.invI)
+                  && a.iso_I == b.iso_I && a.iso_invI == b.iso_invI))
+                return false;
+            const JointK &p = robot::joint[m], &q = robot::joint[3 * l + m];
+            if (!(p.lo == q.lo && p.hi == q.hi && p.kp == q.kp && p.kd == q.kd && p.tmax == q.tmax
+                  && p.wide == q.wide && p.has_limits == q.has_limits && p.lim[0] == q.lim[0]
+                  && p.lim[1] == q.lim[1] && p.lim[2] == q.lim[2] && p.lim[3] == q.lim[3]))
+                return false;
+        }
+        for (int k = 1; k <= 5; ++k)
+            if (!(same_v3(robot::sphere[k].at, robot::sphere[5 * l + k].at)
+                  && robot::sphere[k].rad == robot::sphere[5 * l + k].rad))
+                return false;
+    }
+    return true;
+}
+static_assert(legs_alike(), "the leg form needs legs that differ only in joint geometry");
+
 /* value of lane T of each quad, in all four lanes (DPP quad_perm) */
 template <int T> __device__ __forceinline__ float qbcast(float v)
 {

@@ -38,6 +38,77 @@ struct Rng {
     XD float range(float lo, float hi) { return lo + (hi - lo) * ((float)next() * 2.3283064365386963e-10f); }
 };
 
+/* xorshift128 is linear over GF(2)^128: the state after k draws is a fixed
+ * 128 x 128 bit matrix times the state.  A matrix is kept as the images of
+ * the 128 unit states (bit j = word j / 32, bit j % 32), so applying it is an
+ * XOR of the columns selected by the state's set bits. */
+struct RngMatrix {
+    uint32_t col[128][4];
+};
+XD void rng_apply(const RngMatrix &m, uint32_t s[4])
+{
+    uint32_t r0 = 0u, r1 = 0u, r2 = 0u, r3 = 0u;
+    for (int j = 0; j < 128; ++j) {
+        const uint32_t mask = 0u - ((s[j >> 5] >> (j & 31)) & 1u);
+        r0 ^= m.col[j][0] & mask;
+        r1 ^= m.col[j][1] & mask;
+        r2 ^= m.col[j][2] & mask;
+        r3 ^= m.col[j][3] & mask;
+    }
+    s[0] = r0;
+    s[1] = r1;
+    s[2] = r2;
+    s[3] = r3;
+}
+/* the matrix of one draw, and products of matrices (a after b) */
+inline void rng_step_matrix(RngMatrix &m)
+{
+    for (int j = 0; j < 128; ++j) {
+        Rng r;
+        r.s[0] = r.s[1] = r.s[2] = r.s[3] = 0u;
+        r.s[j >> 5] = 1u << (j & 31);
+        r.next();
+        for (int w = 0; w < 4; ++w)
+            m.col[j][w] = r.s[w];
+    }
+}
+inline void rng_compose(const RngMatrix &a, const RngMatrix &b, RngMatrix &out)
+{
+    RngMatrix t;
+    for (int j = 0; j < 128; ++j) {
+        uint32_t v[4] = { b.col[j][0], b.col[j][1], b.col[j][2], b.col[j][3] };
+        rng_apply(a, v);
+        for (int w = 0; w < 4; ++w)
+            t.col[j][w] = v[w];
+    }
+    out = t;
+}
+/* the matrix of k draws */
+inline void rng_power(uint64_t k, RngMatrix &out)
+{
+    RngMatrix p, r;
+    rng_step_matrix(p);
+    bool any = false;
+    while (k) {
+        if (k & 1u) {
+            if (any)
+                rng_compose(p, r, r);
+            else
+                r = p;
+            any = true;
+        }
+        k >>= 1;
+        if (k)
+            rng_compose(p, p, p);
+    }
+    if (!any) { /* identity */
+        for (int j = 0; j < 128; ++j)
+            for (int w = 0; w < 4; ++w)
+                r.col[j][w] = (w == (j >> 5)) ? 1u << (j & 31) : 0u;
+    }
+    out = r;
+}
+
 /* value-noise lattice sizes for a grid of n columns per side */
 XD int lattice1(int n) { return n / 8 + 2; }
 XD int lattice2(int n) { return n / 3 + 2; }
@@ -75,6 +146,43 @@ XD void reset_draws(uint32_t seed, bool gen, int n, float *lat, float *jit, floa
     draws[NJ] = r.range(-3.14159265358979323846f, 3.14159265358979323846f);
 }
 
+/* Draws of a reset spread over `lanes` lanes: lane L makes the draws
+ * [L * chunk, (L + 1) * chunk) of the stream, starting from the seeded state
+ * moved ahead by the matrices jump[b] = (chunk * 2^b draws), b = 0..5.
+ * Writes each draw where reset_draws puts it. */
+constexpr int JUMP_BITS = 6;
+XD int reset_draw_count(bool gen, int n)
+{
+    const int m1 = lattice1(n), m2 = lattice2(n);
+    return (gen ? m1 * m1 + m2 * m2 + n * n : 0) + NJ + 1;
+}
+XD void reset_draws_lane(uint32_t seed, bool gen, int n, const RngMatrix *jump, int chunk, int lane, float *lat,
+                         float *jit, float *draws)
+{
+    const int m1 = lattice1(n), m2 = lattice2(n);
+    const int nl = gen ? m1 * m1 + m2 * m2 : 0, nj = gen ? n * n : 0;
+    const int total = nl + nj + NJ + 1;
+    const int a = lane * chunk;
+    if (a >= total)
+        return;
+    Rng r;
+    r.seed(seed);
+    for (int b = 0; b < JUMP_BITS; ++b)
+        if ((lane >> b) & 1)
+            rng_apply(jump[b], r.s);
+    const int e = a + chunk < total ? a + chunk : total;
+    for (int q = a; q < e; ++q) {
+        if (q < nl) {
+            lat[q] = r.range(-1.0f, 1.0f);
+        } else if (q < nl + nj) {
+            jit[q - nl] = r.range(-1.0f, 1.0f);
+        } else {
+            const int k = q - nl - nj;
+            draws[k] = k < NJ ? r.range(-0.1f, 0.1f) : r.range(-3.14159265358979323846f, 3.14159265358979323846f);
+        }
+    }
+}
+
 /* quantisation step F of column c before the re-basing: hills + bumps +
  * jitter, h = 0.03f * F with F = floor(h / 0.03 + 0.5) (F is small: |F| <= 21 at
  * difficulty 1) */
@@ -87,6 +195,33 @@ XD float column_step(const float *lat, int n, float difficulty, float jitter, in
     h = h + (0.12f * difficulty) * noise_at(lat + m1 * m1, m2, 3, ix, iy);
     h = h + 0.06f * difficulty * jitter;
     return floorf(h / 0.03f + 0.5f);
+}
+
+/* the samples of ground_under along one axis: v_0 = -reach, v_k+1 = v_k +
+ * half a cell while v <= reach (the reference's float loop) */
+XD int ground_samples(float reach, float cell)
+{
+    int k = 0;
+    for (float v = -reach; v <= reach; v += 0.5f * cell)
+        ++k;
+    return k;
+}
+XD float ground_sample(float reach, float cell, int k)
+{
+    float v = -reach;
+    for (int i = 0; i < k; ++i)
+        v += 0.5f * cell;
+    return v;
+}
+/* the maximum over samples [p0, p0 + step, ...) of the K x K grid (x outer) */
+XD float ground_part(const TerrainK &t, const TerrainRef &tr, float footprint, float cell, int p0, int step)
+{
+    const float reach = footprint + 0.1f;
+    const int K = ground_samples(reach, cell);
+    float ground = -1e30f;
+    for (int p = p0; p < K * K; p += step)
+        ground = maxf_(ground, terrain_height(t, tr, ground_sample(reach, cell, p / K), ground_sample(reach, cell, p % K)));
+    return ground;
 }
 
 /* highest column under the robot's footprint, csrc/arc_sim.c:104-107 */

@@ -196,3 +196,45 @@ extern "C" void hc_joint_consts(const HC *h, int j, double *o)
     }
     o[18] = k.qrel0.w; o[19] = k.qrel0.x; o[20] = k.qrel0.y; o[21] = k.qrel0.z;
 }
+
+/* The reset's draws made as 64 jump-ahead chunks (the device's way) against
+ * the one sequential stream, and the split footprint scan against the plain
+ * one.  Returns the number of differing values (0 expected). */
+extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
+{
+    const ResetK &R = h->su.rk;
+    const Model &M = h->su.model;
+    const int n = R.grid, cells = n * n;
+    std::vector<float> lat(LAT_LIMIT, 0.0f), jit(cells, 0.0f), lat2(LAT_LIMIT, 0.0f), jit2(cells, 0.0f);
+    float d1[NJ + 1], d2[NJ + 1];
+    reset_draws(seed, gen != 0, n, lat.data(), jit.data(), d1);
+    if (gen)
+        for (int lane = 0; lane < 64; ++lane)
+            reset_draws_lane(seed, true, n, R.jump, R.chunk, lane, lat2.data(), jit2.data(), d2);
+    else
+        reset_draws(seed, false, n, lat2.data(), jit2.data(), d2);
+    int bad = 0;
+    for (int c = 0; c < LAT_LIMIT; ++c)
+        bad += memcmp(&lat[c], &lat2[c], 4) != 0;
+    for (int c = 0; c < cells; ++c)
+        bad += memcmp(&jit[c], &jit2[c], 4) != 0;
+    for (int c = 0; c <= NJ; ++c)
+        bad += memcmp(&d1[c], &d2[c], 4) != 0;
+    /* footprint scan over a world's terrain built from these draws */
+    std::vector<int8_t> q((size_t)(n + 2 * TPAD) * (n + 2 * TPAD), (int8_t)Q_OUT);
+    const int pitch = n + 2 * TPAD;
+    for (int c = 0; c < cells; ++c) {
+        float F = gen ? column_step(lat.data(), n, 0.7f, jit[c], c) : 0.0f;
+        q[(c % n + TPAD) + (c / n + TPAD) * pitch] = (int8_t)(int)F;
+    }
+    TerrainRef tl;
+    tl.q = q.data() + TPAD + TPAD * pitch;
+    tl.pitch = pitch;
+    tl.base = 0.09f;
+    tl.outside = -100.0f;
+    float g1 = ground_under(M.terr, tl, R.footprint, R.cell), g2 = -1e30f;
+    for (int lane = 0; lane < 64; ++lane)
+        g2 = maxf_(g2, ground_part(M.terr, tl, R.footprint, R.cell, lane, 64));
+    bad += memcmp(&g1, &g2, 4) != 0;
+    return bad;
+}

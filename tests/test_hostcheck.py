@@ -77,3 +77,17 @@ def test_unstable_world_frozen():
         o.step(t[k])
         h.step(t[k])
         assert same(h.states(), o.states()).all()
+
+
+@pytest.mark.parametrize("grid,cell", [(64, 0.4), (48, 0.4), (20, 0.25), (8, 0.4)])
+def test_reset_split_matches_sequential(grid, cell):
+    """The device's reset: draws made by 64 lanes jumping ahead in the
+    xorshift128 stream (GF(2) matrix powers), and the footprint scan split
+    over the lanes, equal the sequential stream and scan bit for bit."""
+    import ctypes
+    h = HostCheck(1, grid=grid, cell=cell)
+    h.L.hc_reset_split_check.restype = ctypes.c_int
+    h.L.hc_reset_split_check.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]
+    for seed in (0, 1, 12345, 4294967295):
+        for gen in (1, 0):
+            assert h.L.hc_reset_split_check(h.p, seed, gen) == 0, (seed, gen)
