@@ -1,0 +1,83 @@
+"""100-step rollout aggregates: the device engine against the oracle in
+binary32 (same arithmetic) and in binary64 (the reference's own precision).
+
+Single worlds are chaotic: last-bit differences grow ~1e2-1e4x per step, so
+after 100 steps individual trajectories no longer match (the FP64 and FP32
+oracles diverge from each other the same way).  What must agree are the
+statistics of the ensemble.  Tolerance, per aggregate: the difference of the
+two ensemble means within 4 standard errors of that difference plus 1e-3
+absolute.  The test prints each aggregate's difference against both oracles
+next to the difference between the two oracles themselves."""
+
+import numpy as np
+import pytest
+
+import pyoracle as P
+from helpers import action_stream
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS = 256, 100
+
+
+def _aggregates(root, joints, contacts, flags):
+    """per-world quantities whose ensemble means are compared"""
+    return {
+        "torso_height": root[:, 2],
+        "torso_speed": np.linalg.norm(root[:, 7:10], axis=1),
+        "torso_spin": np.linalg.norm(root[:, 10:13], axis=1),
+        "joint_angle_abs": np.abs(joints[:, :12]).mean(1),
+        "joint_rate_abs": np.abs(joints[:, 12:24]).mean(1),
+        "joint_torque_abs": np.abs(joints[:, 24:36]).mean(1),
+        "foot_contact": contacts[:, :4].mean(1),
+        "torso_contact": contacts[:, 4],
+        "unstable": flags.astype(np.float64),
+    }
+
+
+def _run_oracle(kind, diff, ts):
+    o = P.Lib(kind).sim(N)
+    ids = np.arange(N, dtype=np.int32)
+    o.reset(ids, ids.astype(np.uint32), diff)
+    for t in ts:
+        o.step(t)
+    root, rot, joints, contacts, flags = o.observe()
+    return _aggregates(root, joints, contacts, flags)
+
+
+@pytest.mark.parametrize("form", ["lane", "leg"])
+def test_rollout_stats(form):
+    from arc import native
+    diff = P.difficulties(1, N)
+    ts = action_stream(N, STEPS, seed=11)
+    g = native.Sim(N)
+    g.set_form(form)
+    ids = np.arange(N, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), diff)
+    dev = native.DeviceBuffer.from_array(ts.astype(np.float32))
+    per = N * 12 * 4
+    for k in range(STEPS):
+        g.step_device(dev.ptr + k * per)
+    g.sync()
+    root, joints, contacts, flags = g.observe()
+    a = _aggregates(root, joints, contacts, flags)
+    o = P.Lib("o32").sim(N)
+    o.reset(ids, ids.astype(np.uint32), diff)
+    for t in ts:
+        o.step(t)
+    exact = float(np.mean(np.all(o.observe()[0] == root, axis=1)))
+    print(f"{form}: worlds whose root state equals the binary32 oracle bit for bit after {STEPS} steps: {exact:.3f}")
+    o32 = _run_oracle("o32", diff, ts)
+    o64 = _run_oracle("o64", diff, ts)
+    bad = []
+    for key in a:
+        m = {name: float(np.mean(v[key])) for name, v in (("gpu", a), ("o32", o32), ("o64", o64))}
+        for ref_name, ref in (("o32", o32), ("o64", o64)):
+            se = np.sqrt(np.var(a[key]) / N + np.var(ref[key]) / N)
+            d = abs(m["gpu"] - m[ref_name])
+            if d > 4.0 * se + 1e-3:
+                bad.append((key, ref_name, d, se))
+        print(f"{form} {key:18s} gpu {m['gpu']:.5g}  o32 {m['o32']:.5g}  o64 {m['o64']:.5g}  "
+              f"|gpu-o32| {abs(m['gpu'] - m['o32']):.3g}  |gpu-o64| {abs(m['gpu'] - m['o64']):.3g}  "
+              f"|o32-o64| {abs(m['o32'] - m['o64']):.3g}")
+    assert not bad, bad
