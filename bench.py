@@ -122,6 +122,24 @@ def batched_pass(steps: int, warm: int) -> dict:
             "resets": int(stats[1:].sum()), "bytes_per_world": s.bytes_per_world()}
 
 
+def abi_pass(steps: int, warm: int) -> dict:
+    """arc_sim_step through the reference ABI: host float64 targets in, the
+    step, return on completion (PCIe-inclusive; never the headline)"""
+    from arc import native
+    s = new_sim(N)
+    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+    cfg = warm_and_tune(s, N, t, warm)
+    host = [actions_to_targets(s.default_angles, rng(50 + k, N * NJ)).astype(np.float64).reshape(N, NJ)
+            for k in range(4)]
+    s.step(host[0])
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step(host[k % 4])
+    el = time.perf_counter() - t0
+    return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "launch": cfg,
+            "path": "arc_sim_step(host float64 targets): convert, copy to HBM, step, synchronize"}
+
+
 def plain_pass(steps: int = 3) -> dict:
     from arc import native
     s = new_sim(N)
@@ -346,7 +364,9 @@ def main() -> int:
     log(f"served {served['value']:.4g} world-steps/s (closed loop {served['closed_loop']['value']:.4g}); "
         "batched graph form")
     batched = batched_pass(steps, warm)
-    log(f"batched {batched['value']:.4g}; kernel pass")
+    log(f"batched {batched['value']:.4g}; reference ABI path")
+    abi = abi_pass(10, warm)
+    log(f"abi {abi['value']:.4g}; kernel pass")
     kp = kernel_pass(steps, warm)
     log(f"{kernel_name(kp['launch'])} {kp['kernel_ms_avg']:.3f} ms; plain form")
     plain = plain_pass()
@@ -383,7 +403,7 @@ def main() -> int:
                      "kernel": kname, "kernel_ms_avg": kp["kernel_ms_avg"], "worlds_per_launch": N,
                      "flop_per_world_step": FLOP_PER_WORLD_STEP,
                      "hbm_frac": N * BYTES_PER_WORLD_STEP / avg_s / PEAK_HBM},
-        "forms": {"served": served, "batched_graph": batched, "plain": plain},
+        "forms": {"served": served, "batched_graph": batched, "plain": plain, "abi_pcie_inclusive": abi},
         "latency_4096": lat,
         "serving": serving,
         "launch": kp["launch"],
