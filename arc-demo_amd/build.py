@@ -27,8 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", 
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-fno-gpu-rdc"]
 
-HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip"]
-HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp"]
+HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip"]
+HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "task.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp"]
 
 
 def hipcc() -> str:

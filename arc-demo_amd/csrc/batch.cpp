@@ -186,6 +186,18 @@ static hipError_t graph_for(arc_sim *s, BatchState *b, const LaunchKey &k, hipGr
     return hipSuccess;
 }
 
+/* buffers the configured launch allocates on first use, made now (before a
+ * graph capture, which must not allocate) */
+hipError_t prepare_launch(arc_sim *s)
+{
+    hipError_t e;
+    if (is_leg(s) && (e = ensure_leg(s)) != hipSuccess)
+        return e;
+    if (s->persistent && !batch_of(s))
+        return hipErrorOutOfMemory;
+    return hipSuccess;
+}
+
 /* every world of the sim, lane or leg form, in the configured distribution */
 hipError_t launch_lane_all(arc_sim *s, const float *targets)
 {

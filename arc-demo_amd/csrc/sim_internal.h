@@ -60,6 +60,8 @@ void batch_free(arc_sim *s);
 hipError_t launch_lane_all(arc_sim *s, const float *targets);
 /* the leg form's contact lists (allocated on first use) */
 hipError_t ensure_leg(arc_sim *s);
+/* buffers the configured launch allocates on first use, made before a capture */
+hipError_t prepare_launch(arc_sim *s);
 
 #define HIPCHK(s, call, what)                                 \
     do {                                                      \

@@ -1,0 +1,88 @@
+/*
+ * arc_task.h -- the checkpoint-reaching task of the reference (arc/task.py,
+ * arc/env.py) run on the device over an arc_sim's worlds.
+ *
+ * The reference computes observations, rewards, terminations, checkpoint
+ * placement and the terrain curriculum in numpy over all worlds after every
+ * arc_sim_step (arc/task.py:155-280), then the vectorised env resets the
+ * ended worlds and re-observes them (arc/env.py:99-117).  Here one call does
+ * all of it on the GPU, with the actions read from and the observations
+ * written to device memory, so a policy on the same GPU exchanges them
+ * without host copies.  Differences: checkpoint placement and reset seeds
+ * come from a per-world stream of the engine's xorshift128 generator (the
+ * reference draws them from one numpy PCG64 generator); the rules, ranges and
+ * order of the draws' uses are the reference's.
+ */
+#ifndef ARC_TASK_H
+#define ARC_TASK_H
+
+#include "arc_sim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct arc_task arc_task;
+
+/* TaskConfig of arc/task.py:19-62 (the simulation fields come from the sim) */
+typedef struct arct_config {
+    int max_steps;
+    double action_scale;
+    double checkpoint_radius, checkpoint_min_dist, checkpoint_max_dist, border;
+    double w_progress, max_speed, fov_deg, w_turn, w_checkpoint, w_time, w_fall, w_energy, w_action_rate,
+        w_body_contact, w_torso_contact, w_body_rate, w_bounce;
+    double flip_up_z, void_z;
+    int beached_steps;
+    int curriculum;
+    double difficulty, max_difficulty, level_step;
+    int level_up_checkpoints, level_down_steps;
+} arct_config;
+
+/* the reference's defaults */
+void arct_default_config(arct_config *c);
+
+/* A task over every world of s (which it steps; s must outlive it).
+ * offsets2k: the k (forward, left) height samples around the torso (the
+ * reference's 13 x 11 grid, arc/task.py:73-76).  seed: the per-world streams.
+ * NULL on failure. */
+arc_task *arct_create(arc_sim *s, const arct_config *c, const double *offsets2k, int k, unsigned int seed);
+void arct_destroy(arc_task *t);
+
+/* floats per observation: 3 + 3 + 3 + 3 x 12 + 3 + k + 4 (195 for k = 143) */
+int arct_obs_size(const arc_task *t);
+
+/* Device buffers owned by the task (valid until arct_destroy):
+ *   0 observations        float   [N][obs]
+ *   1 terminal obs        float   [N][obs]  rows of worlds that ended in the last step
+ *   2 rewards             float   [N]
+ *   3 terminated          uint8   [N]
+ *   4 truncated           uint8   [N]
+ *   5 info                int32   [3][N]    cause (0 none, 1 unstable, 2 void, 3 flip, 4 beached),
+ *                                           checkpoints, steps of the episode that just ran
+ *   6 episode difficulty  float64 [N]
+ *   7 actions             float   [N][12]   the default input of arct_step */
+void *arct_buffer(arc_task *t, int which);
+
+/* Resets every world (seeds from the streams, difficulty = level) and
+ * writes the observations.  Asynchronous on the sim's stream. */
+int arct_reset_all(arc_task *t);
+/* One env step: actions (device float [N][12]; NULL = buffer 7) -> drive
+ * targets, the step, rewards / terminations / checkpoints / curriculum,
+ * observations, terminal observations of the ended worlds, their reset and
+ * their new first observations.  graph = 1 replays a captured hipGraph of
+ * the sequence (captured again when actions or the sim's launch
+ * configuration change).  Asynchronous; arct_sync waits. */
+int arct_step(arc_task *t, const float *actions_dev, int graph);
+int arct_sync(arc_task *t);
+
+/* host read-backs: checkpoints (N x 3: x, y, terrain z), curriculum levels
+ * (N), and the worlds and seeds of the last resets (returns their count) */
+int arct_targets(const arc_task *t, double *out3);
+int arct_levels(const arc_task *t, double *out);
+int arct_last_resets(const arc_task *t, int *ids, unsigned int *seeds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

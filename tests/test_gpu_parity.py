@@ -5,7 +5,7 @@ compiled in FP32, see test_oracle_ref.py).  The device engine evaluates the
 same binary32 operations in the same order; only the binary64 atan2 / sin /
 cos of the device library may round differently in the last bit from glibc.
 
-Bars (north star): per-step body poses and velocities within 1e-5 relative
+Bars: per-step body poses and velocities within 1e-5 relative
 after one control step (|a - b| <= 1e-5 (|b| + 1), per field, the +1 guarding
 fields near zero); flags, contact counts, reset terrain and RNG streams
 bit-exact; longer rollouts compared through statistics (test_rollout_stats).

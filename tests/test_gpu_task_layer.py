@@ -1,0 +1,90 @@
+"""GPU task layer (include/arc_task.h): observations, rewards, terminations,
+checkpoints, curriculum and the env's auto-reset order, against the numpy
+restatement of arc/task.py and arc/env.py (oracle/pytask.py) over the
+binary32 oracle sim.
+
+The reference draws reset seeds and checkpoint positions from numpy's PCG64;
+the device draws them from per-world xorshift128 streams.  The test reads the
+device's draws back and gives them to the oracle, which restates everything
+else.  The engine state is bit-exact between the two, the task arithmetic is
+binary64 on both sides (only the device's and glibc's atan2 / cos / sin /
+hypot may differ in the last bit), so: integers (terminations, causes,
+checkpoint counts, episode steps, reset lists) and curriculum levels exact;
+rewards and observations (float32) within 1e-5 (|b| + 1).  The tightened
+config makes checkpoints, falls, truncations and level changes frequent."""
+
+import numpy as np
+import pytest
+
+import pyoracle as P
+from helpers import action_stream
+from pytask import OracleTask
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _close(a, b):
+    return np.all(np.abs(a.astype(np.float64) - b.astype(np.float64)) <= TOL * (np.abs(b) + 1.0))
+
+
+@pytest.mark.parametrize("form", ["lane", "leg"])
+def test_device_task_matches_oracle(form):
+    from arc import devtask
+    n, steps = 96, 40
+    cfg = devtask.TaskConfig(max_steps=17, checkpoint_radius=3.0, fov=120.0, beached_steps=2, flip_up_z=0.95,
+                             level_up_checkpoints=1, level_down_steps=10)
+    t = devtask.DeviceTask(n, cfg, seed=3, form=form)
+    t.reset()
+    t.sync()
+    ids, seeds = t.last_resets()
+    assert np.array_equal(ids, np.arange(n))
+    tg = t.targets()
+    o = OracleTask(P.Lib("o32").sim(n, cfg.dt, cfg.substeps, cfg.grid, cfg.cell), cfg, devtask.HEIGHT_OFFSETS,
+                   cfg.grid, cfg.cell)
+    o.reset(ids, seeds, tg[ids, :2])
+    assert np.array_equal(o.target_z, tg[:, 2])
+    assert _close(t.obs.numpy(), o.observe())
+    # first checkpoints: 3-6 m from the spawn point, inside the border
+    limit = 0.5 * cfg.grid * cfg.cell - cfg.border
+    r = np.hypot(tg[:, 0] - o.root[:, 0], tg[:, 1] - o.root[:, 1])
+    assert np.all((r >= 3.0 - 1e-9) & (r <= 6.0 + 1e-9)) and np.all(np.abs(tg[:, :2]) < limit)
+
+    # the action stream's uniform actions, scaled by 1.3 so that the clip to [-1, 1] acts too
+    acts = action_stream(n, steps, seed=21)
+    raw = ((np.asarray(acts) - o.default_angles) / 0.6 * 1.3).astype(np.float32)
+    counts = {"reached": 0, "fell": 0, "truncated": 0, "level_changes": 0}
+    for k in range(steps):
+        a = raw[k]
+        t.actions_upload(a)
+        t.step(graph=(k % 2 == 0))
+        t.sync()
+        tg = t.targets()
+        lv_before = o.level.copy()
+        rd, term, trunc, stats, hit = o.step(a, lambda h: tg[h, :2])
+        obs = o.observe()
+        ended = np.flatnonzero(term | trunc)
+        dids, dseeds = t.last_resets()
+        assert np.array_equal(dids, ended), f"step {k}: reset lists"
+        o.reset(ended, dseeds, tg[ended, :2])
+        obs2 = o.observe()
+        final = obs.copy()
+        final[ended] = obs2[ended]
+        assert np.array_equal(t.terminated.numpy().astype(bool), term), f"step {k}"
+        assert np.array_equal(t.truncated.numpy().astype(bool), trunc), f"step {k}"
+        info = t.info.numpy()
+        assert np.array_equal(info[0], stats["cause"]) and np.array_equal(info[1], stats["checkpoints"])
+        assert np.array_equal(info[2], stats["steps"])
+        assert np.array_equal(t.difficulty.numpy(), stats["difficulty"])
+        assert _close(t.reward.numpy(), rd), f"step {k}: rewards"
+        assert _close(t.terminal_obs.numpy()[ended], obs[ended]), f"step {k}: terminal observations"
+        assert _close(t.obs.numpy(), final), f"step {k}: observations"
+        assert np.array_equal(t.levels(), o.level), f"step {k}: levels"
+        counts["reached"] += len(hit)
+        counts["fell"] += int(term.sum())
+        counts["truncated"] += int(trunc.sum())
+        counts["level_changes"] += int(np.sum(o.level != lv_before))
+    print(form, counts)
+    assert counts["reached"] > 0 and counts["fell"] > 0 and counts["truncated"] > 0 and counts["level_changes"] > 0
+    t.close()
