@@ -99,6 +99,7 @@ def _task_lib():
             ("arct_sync", _int, [_vp]),
             ("arct_targets", _int, [_vp, _vp]),
             ("arct_levels", _int, [_vp, _vp]),
+            ("arct_set_levels", _int, [_vp, _vp]),
             ("arct_last_resets", _int, [_vp, _vp, _vp]),
         ]:
             fn = getattr(lib, name)
@@ -247,6 +248,12 @@ class DeviceTask:
         out = np.zeros((self.num_worlds, 3))
         self._lib.arct_targets(self._ptr, out.ctypes.data)
         return out
+
+    def set_levels(self, levels: np.ndarray) -> None:
+        lv = np.ascontiguousarray(levels, dtype=np.float64)
+        assert lv.shape == (self.num_worlds,)
+        if self._lib.arct_set_levels(self._ptr, lv.ctypes.data):
+            self.sim._check()
 
     def levels(self) -> np.ndarray:
         out = np.zeros(self.num_worlds)

@@ -300,6 +300,15 @@ int arct_targets(const arc_task *ct, double *out3)
     return 0;
 }
 
+int arct_set_levels(arc_task *t, const double *levels)
+{
+    arc_sim *s = t->sim;
+    HIPCHK_RET(s, hipMemcpyAsync(t->td.level, levels, (size_t)t->n * sizeof(double), hipMemcpyHostToDevice, s->stream),
+               "arct_set_levels", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arct_set_levels", -1);
+    return 0;
+}
+
 int arct_levels(const arc_task *ct, double *out)
 {
     arc_task *t = const_cast<arc_task *>(ct);

@@ -281,6 +281,64 @@ def kernel_name(launch: dict) -> str:
     return "k_step_leg" if launch.get("form") == "leg" else "k_step"
 
 
+def rollout_pass(steps: int = 64, rollouts: int = 3, n: int = 8192) -> dict:
+    """8,192 worlds at difficulty U[0,1), 64-step rollouts: a random-init
+    2 x 256 tanh MLP in PyTorch reads the device task's observations and
+    writes its actions, both exchanged zero-copy (DLPack out, device pointer
+    in); rewards and dones gathered into a rollout buffer on the GPU.  Runs in
+    its own process (torch's HIP runtime must load before libarcsim)."""
+    import torch  # noqa: F401  (first: libarcsim then binds torch's HIP runtime)
+    from arc import devtask
+    cfg = devtask.TaskConfig(curriculum=False)
+    t = devtask.DeviceTask(n, cfg, seed=0)
+    t.set_levels(unit24(rng(1, n)))
+    t.reset()
+    cfgl, _ = t.sim.autotune(3)
+    dev = torch.device("cuda")
+    torch.manual_seed(0)
+    policy = torch.nn.Sequential(torch.nn.Linear(t.obs_size, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
+                                 torch.nn.Tanh(), torch.nn.Linear(256, 12)).to(dev)
+    obs = torch.from_dlpack(t.obs)            # the task's observation buffer, no copy
+    rew = torch.from_dlpack(t.reward)
+    done_a, done_b = torch.from_dlpack(t.terminated), torch.from_dlpack(t.truncated)
+    buf_obs = torch.empty((steps, n, t.obs_size), device=dev)
+    buf_rew = torch.empty((steps, n), device=dev)
+    buf_done = torch.empty((steps, n), dtype=torch.bool, device=dev)
+    cur = torch.cuda.current_stream()
+
+    def rollout():
+        with torch.no_grad():
+            for k in range(steps):
+                buf_obs[k].copy_(obs)
+                act = policy(obs)
+                cur.synchronize()                  # actions complete before the task reads them
+                t.step(act, graph=True)
+                t.sync()                           # observations complete before torch reads them
+                buf_rew[k].copy_(rew)
+                buf_done[k].copy_((done_a | done_b).bool())
+        cur.synchronize()
+
+    rollout()  # warm (graph capture, kernels)
+    t0 = time.perf_counter()
+    for _ in range(rollouts):
+        rollout()
+    el = time.perf_counter() - t0
+    return {"value": n * steps * rollouts / el, "unit": "world-steps/s", "worlds": n, "rollout_steps": steps,
+            "rollouts": rollouts, "ms_per_step": 1e3 * el / (steps * rollouts), "launch": cfgl,
+            "obs_size": t.obs_size, "mean_reward": float(buf_rew.mean()), "done_fraction": float(buf_done.float().mean()),
+            "policy": "random-init MLP 195-256-256-12 tanh, fp32", "exchange": "DLPack (obs, rewards, dones), device pointer (actions)"}
+
+
+def rollout_child() -> dict:
+    import subprocess
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--only", "rollout"], capture_output=True,
+                       text=True, timeout=600)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode or not lines:
+        return {"value": None, "error": (r.stderr or r.stdout)[-800:]}
+    return json.loads(lines[-1])["rollout"]
+
+
 def traffic_from_profile(kernel: str):
     """HBM bytes per launch of `kernel` at 65,536 worlds from the committed
     counter summary (FETCH_SIZE / WRITE_SIZE, calibrated), or None."""
@@ -336,7 +394,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--latency-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "kernel"], default="all")
+    ap.add_argument("--only", choices=["all", "kernel", "rollout"], default="all")
+    ap.add_argument("--no-rollout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested): worlds would shard by "
@@ -348,6 +407,9 @@ def main() -> int:
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.only == "rollout":
+        print(json.dumps({"rollout": rollout_pass()}), flush=True)
+        return 0
     if args.only == "kernel":
         # a plain 1 GiB device copy first: the counters' reading for known traffic
         from arc import native
@@ -375,6 +437,10 @@ def main() -> int:
     log(f"latency p50 {lat['p50_ms']:.3f} p99 {lat['p99_ms']:.3f} ms; serving")
     serving = serving_pass(args.serve_seconds)
     log(f"serving p50 {serving['p50_ms']:.3f} p99 {serving['p99_ms']:.3f} ms")
+    rollout = None
+    if not args.no_rollout:
+        rollout = rollout_child()
+        log(f"rollout loop {rollout.get('value')}")
 
     avg_s = kp["kernel_ms_avg"] / 1e3
     achieved = N * FLOP_PER_WORLD_STEP / avg_s
@@ -406,6 +472,7 @@ def main() -> int:
         "forms": {"served": served, "batched_graph": batched, "plain": plain, "abi_pcie_inclusive": abi},
         "latency_4096": lat,
         "serving": serving,
+        "rollout_8192": rollout,
         "launch": kp["launch"],
     }
     if not args.no_cpu:

@@ -75,6 +75,10 @@ int arct_reset_all(arc_task *t);
 int arct_step(arc_task *t, const float *actions_dev, int graph);
 int arct_sync(arc_task *t);
 
+/* curriculum levels of every world (N doubles, host): the difficulty of each
+ * world's next episode (with curriculum off, its fixed difficulty) */
+int arct_set_levels(arc_task *t, const double *levels);
+
 /* host read-backs: checkpoints (N x 3: x, y, terrain z), curriculum levels
  * (N), and the worlds and seeds of the last resets (returns their count) */
 int arct_targets(const arc_task *t, double *out3);
