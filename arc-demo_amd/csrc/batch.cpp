@@ -102,7 +102,12 @@ static BatchState *batch_of(arc_sim *s)
     return b;
 }
 
-static int is_leg(const arc_sim *s) { return s->form == 2; }
+/* the group forms: 1 = four lanes per world (one leg each), 2 = two lanes
+ * per world (two legs each); 0 = one world per lane */
+static int is_leg(const arc_sim *s) { return s->form == 2 ? 1 : s->form == 3 ? 2 : 0; }
+static int lanes_per_world(const arc_sim *s) { return s->form == 2 ? 4 : s->form == 3 ? 2 : 1; }
+/* LDS layout actually launched: the group forms always hold the terrain tile */
+static int tile_of(const arc_sim *s) { return is_leg(s) ? 16 : s->tile; }
 
 /* the leg form's per-lane contact lists, allocated on first use */
 hipError_t ensure_leg(arc_sim *s)
@@ -110,20 +115,20 @@ hipError_t ensure_leg(arc_sim *s)
     if (s->dev.lcbuf)
         return hipSuccess;
     const Model &M = s->su.model;
-    /* the form records contacts per lane without the per-world running
-     * total: it needs every sphere window to fit the 128-contact cap */
+    /* the forms record contacts per lane without the per-world running
+     * total: they need every sphere window to fit the 128-contact cap */
     if (M.cslots > MAXC)
         return hipErrorNotSupported;
-    const size_t slots = (size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]);
-    return hipMalloc((void **)&s->dev.lcbuf, slots * 6 * 4 * s->dev.stride * sizeof(float));
+    const size_t slots = (size_t)std::max(leg_world_slots(M, 1), leg_world_slots(M, 2));
+    return hipMalloc((void **)&s->dev.lcbuf, slots * 6 * s->dev.stride * sizeof(float));
 }
 
 /* grid of a launch over `total` worlds (graph: the worst case N) */
 static int grid_for(const arc_sim *s, int total)
 {
-    const int items = (is_leg(s) ? 4 : 1) * total;
+    const int items = lanes_per_world(s) * total;
     if (s->persistent) {
-        int r = step_resident_blocks(s->block, s->tile, is_leg(s));
+        int r = step_resident_blocks(s->block, tile_of(s), is_leg(s));
         return r > 0 ? r : (items + s->block - 1) / s->block;
     }
     if (s->grid_blocks > 0)
@@ -137,7 +142,7 @@ static hipError_t enqueue(arc_sim *s, BatchState *b, const float *targets, int t
     hipError_t e;
     if (s->persistent && (e = launch_fill_int(b->d_work, 0, 1, s->stream)) != hipSuccess)
         return e;
-    if ((e = launch_step(s->dev, s->d_model, targets, b->d_batch, 0, s->block, s->tile, grid,
+    if ((e = launch_step(s->dev, s->d_model, targets, b->d_batch, 0, s->block, tile_of(s), grid,
                          s->persistent ? b->d_work : nullptr, is_leg(s), s->stream)) != hipSuccess)
         return e;
     if (!task)
@@ -205,14 +210,14 @@ hipError_t launch_lane_all(arc_sim *s, const float *targets)
     if (is_leg(s) && (e = ensure_leg(s)) != hipSuccess)
         return e;
     if (!s->persistent)
-        return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, s->grid_blocks, nullptr,
+        return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, tile_of(s), s->grid_blocks, nullptr,
                            is_leg(s), s->stream);
     BatchState *b = batch_of(s);
     if (!b)
         return hipErrorOutOfMemory;
     if ((e = launch_fill_int(b->d_work, 0, 1, s->stream)) != hipSuccess)
         return e;
-    return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, s->tile, grid_for(s, s->n), b->d_work,
+    return launch_step(s->dev, s->d_model, targets, nullptr, s->n, s->block, tile_of(s), grid_for(s, s->n), b->d_work,
                        is_leg(s), s->stream);
 }
 
@@ -235,8 +240,8 @@ int arcx_set_persistent(arc_sim *s, int on)
 int arcx_get_launch(const arc_sim *s, int *out5)
 {
     out5[0] = s->block;
-    out5[1] = s->tile;
-    out5[2] = s->persistent ? step_resident_blocks(s->block, s->tile, is_leg(s)) : s->grid_blocks;
+    out5[1] = tile_of(s);
+    out5[2] = s->persistent ? step_resident_blocks(s->block, tile_of(s), is_leg(s)) : s->grid_blocks;
     out5[3] = s->persistent;
     out5[4] = s->form;
     return 0;
@@ -305,7 +310,7 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
         HIPCHK_RET(s, enqueue(s, b, targets_dev, task, grid_for(s, std::max(total, 1))), "arcx_step_batch launch", -1);
         return 0;
     }
-    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, s->tile, grid_for(s, std::max(s->n, 1)), s->persistent,
+    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, tile_of(s), grid_for(s, std::max(s->n, 1)), s->persistent,
                  is_leg(s) };
     hipGraphExec_t exec = nullptr;
     HIPCHK_RET(s, graph_for(s, b, k, &exec), "arcx_step_batch capture", -1);
@@ -356,11 +361,11 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
         bool ok = hipMemcpyAsync(dt, ht.data(), nt * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
                   && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
                   && hipStreamSynchronize(s->stream) == hipSuccess;
-        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[2] = { 0, 2 };
+        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[3] = { 0, 2, 3 };
         const bool leg_ok = ensure_leg(s) == hipSuccess; /* else lane form only */
-        for (int fi = 0; fi < 2 && ok; ++fi)
-        for (int bi = 0; bi < 3 && ok && (forms[fi] != 2 || leg_ok); ++bi)
-            for (int ti = 0; ti < 2 && ok; ++ti)
+        for (int fi = 0; fi < 3 && ok; ++fi)
+        for (int bi = 0; bi < 3 && ok && (forms[fi] == 0 || leg_ok); ++bi)
+            for (int ti = 0; ti < (forms[fi] == 0 ? 2 : 1) && ok; ++ti)
                 for (int pi = 0; pi < 2 && ok; ++pi) {
                     s->form = forms[fi];
                     s->block = blocks[bi];

@@ -100,6 +100,13 @@ struct StepBatch {
     int n_ranges, total, pad[2];
     Range r[MAX_BATCH];
 };
+/* contact-list slots per world of the group form with K legs per lane (its
+ * 4 / K lanes each hold the torso's, and their legs' femur and tibia lists) */
+inline __host__ __device__ int leg_world_slots(const Model &M, int K)
+{
+    return (LEGS / K) * (M.ccap[0] + K * (M.ccap[1] + M.ccap[5]));
+}
+
 /* world index of work item t of a batch: binary search over the prefixes */
 __device__ __forceinline__ int batch_world(const StepBatch *__restrict__ b, int n_ranges, int t)
 {
@@ -133,7 +140,8 @@ hipError_t launch_init_template(const DevWorlds &d, const TemplateK *tk_dev, flo
 /* step kernel.  batch (device) or worlds 0..total-1; tile 0 or 16 (LDS
  * terrain tile side); grid_blocks 0 = one work item per thread; work (device
  * counter, zeroed beforehand) selects the persistent distribution; leg = 1:
- * four lanes per world (leg_engine.h; needs d.lcbuf), else one. */
+ * four lanes per world, 2: two lanes per world (leg_engine.h; need d.lcbuf
+ * and tile 16), 0: one. */
 hipError_t launch_step(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch, int total,
                        int block, int tile, int grid_blocks, int *work, int leg, hipStream_t s);
 hipError_t launch_step_b64(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,

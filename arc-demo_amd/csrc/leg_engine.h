@@ -1,23 +1,27 @@
 /*
- * leg_engine.h -- one control step of one quadruped world run by the four
- * lanes of a quad: lane q holds leg q (coxa, femur, tibia) and a replica of
- * the torso, which every lane keeps identical.
+ * leg_engine.h -- one control step of one quadruped world run by a group of
+ * P lanes: each lane holds K = 4 / P legs (coxa, femur, tibia) and a replica
+ * of the torso, which every lane of the group keeps identical.  P = 4 (a
+ * quad, one leg per lane) and P = 2 (a pair, two legs per lane).
  *
  * The reference projects the joints in the order 0..11 (csrc/arc_phys.c:
  * 726-737, Gauss-Seidel).  Joints 3l (hip yaw) couple the torso with leg l;
  * joints 3l+1 and 3l+2 touch only leg l's own bodies.  So the order
- *     root 0, root 1, root 2, root 3, then every leg's joint 1, then joint 2
+ *     root 0, root 1, root 2, root 3, then every leg's joint 1 and joint 2
  * applies the same sequence of updates to every body (operations on disjoint
  * bodies commute exactly) and gives bit-identical results.  The four roots
- * run one after another, each in the lane of its leg, and the torso it
- * produces is broadcast to the quad (DPP) before the next; the mid and distal
- * joints of the four legs run at once.  Spheres, integration and the
- * velocity passes act on one body at a time: each lane does its leg's bodies
- * and the torso's (the same arithmetic in all four lanes).
+ * run one after another, each in the lane holding its leg, and the torso is
+ * broadcast to the group (DPP) when the next root belongs to another lane;
+ * the mid and distal joints of the group's legs run at once.  Spheres,
+ * integration and the velocity passes act on one body at a time: each lane
+ * does its legs' bodies and the torso's (the same arithmetic in every lane).
  *
- * Per lane this is 4 bodies, 6 spheres and 6 joint solves per substep
- * against 13, 21 and 12 in the one-world-per-lane form (lane_engine.h):
- * a shorter dependent chain per world, for latency-bound batches.
+ * With P = 4 a lane does 4 bodies, 6 spheres and 6 joint solves per substep
+ * against 13, 21 and 12 in the one-world-per-lane form (lane_engine.h): a
+ * 2.5x shorter dependent chain per world, for latency-bound batches and, at
+ * two wavefronts per SIMD, the best throughput too.  P = 2 halves the
+ * redundant work but holds 7 bodies per lane; it spills registers and is
+ * slower on gfx950 (kept as a tuned alternative).
  */
 #pragma once
 #include "lane_engine.h"
@@ -51,19 +55,28 @@ constexpr bool legs_alike()
 }
 static_assert(legs_alike(), "the leg form needs legs that differ only in joint geometry");
 
-/* value of lane T of each quad, in all four lanes (DPP quad_perm) */
-template <int T> __device__ __forceinline__ float qbcast(float v)
+/* Cross-lane helpers of a world's group of P lanes (P = 4: a quad; P = 2: a
+ * pair of a quad), DPP quad_perm within each quad. */
+template <int P, int T> __device__ __forceinline__ float gbcast(float v)
 {
-    constexpr int ctrl = T | (T << 2) | (T << 4) | (T << 6);
+    /* lane T of the group, in every lane of the group */
+    constexpr int ctrl = P == 4 ? (T | (T << 2) | (T << 4) | (T << 6))
+                                : (T | (T << 2) | ((2 + T) << 4) | ((2 + T) << 6));
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), ctrl, 0xf, 0xf, false));
 }
-__device__ __forceinline__ int qor(int v)
+template <int P> __device__ __forceinline__ int gor(int v)
 {
     v |= __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false); /* quad_perm 1,0,3,2 */
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false); /* quad_perm 2,3,0,1 */
+    if (P == 4)
+        v |= __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false); /* quad_perm 2,3,0,1 */
     return v;
 }
-template <int T> __device__ __forceinline__ void qbcast3(V3 &a) { a.x = qbcast<T>(a.x); a.y = qbcast<T>(a.y); a.z = qbcast<T>(a.z); }
+template <int P, int T> __device__ __forceinline__ void gbcast3(V3 &a)
+{
+    a.x = gbcast<P, T>(a.x);
+    a.y = gbcast<P, T>(a.y);
+    a.z = gbcast<P, T>(a.z);
+}
 
 /* one of four per-leg values by the lane's leg */
 __device__ __forceinline__ float sel4(int q, float a, float b, float c, float d) { return q == 0 ? a : q == 1 ? b : q == 2 ? c : d; }
@@ -97,66 +110,70 @@ __device__ __forceinline__ JointK leg_joint(int q, const Model &M)
     return J;
 }
 
-/* State of one world in one lane of its quad: the torso replica, the leg's
- * three bodies (0 coxa, 1 femur, 2 tibia) and its three joints; spheres in
- * lane-local order 0 torso, 1 femur, 2..5 tibia. */
-struct LegWorld {
+/* State of one world in one lane of its group: the torso replica, the
+ * lane's K legs (bodies 0 coxa, 1 femur, 2 tibia) and their joints; spheres
+ * in lane-local order 0 torso, then 1 + 5 l + m for sphere m of local leg l. */
+template <int K> struct LegWorld {
     Body T;
-    Body b[3];
-    float target[3], tacc[3], angle[3], rate[3], torque[3];
-    uint32_t touchp[2]; /* byte k of word k / 4: substeps local sphere k touched */
-    uint32_t anchored;  /* bit k: local sphere k has a valid friction anchor */
+    Body b[K][3];
+    float target[3 * K], tacc[3 * K], angle[3 * K], rate[3 * K], torque[3 * K];
+    uint32_t touchp[(1 + 5 * K + 3) / 4]; /* byte k of word k / 4: substeps local sphere k touched */
+    uint32_t anchored;                    /* bit k: local sphere k has a valid friction anchor */
     int unstable;
     float time;
 };
-__device__ __forceinline__ int leg_touch(const LegWorld &W, int k) { return (int)((W.touchp[k >> 2] >> (8 * (k & 3))) & 0xffu); }
-/* global sphere index of local sphere k of leg q */
-__device__ __forceinline__ int leg_sphere(int q, int k) { return k == 0 ? 0 : 5 * q + k; }
-
-/* contact slots of one lane: torso, femur, tibia ranges of sizes ccap[0],
- * ccap[1], ccap[5] (every leg's femur and tibia have the same spheres) */
-__device__ __forceinline__ int leg_slot(const Model &M, int lb)
+template <int K> __device__ __forceinline__ int leg_touch(const LegWorld<K> &W, int k)
 {
-    return lb == 0 ? 0 : lb == 1 ? M.ccap[0] : M.ccap[0] + M.ccap[1];
+    return (int)((W.touchp[k >> 2] >> (8 * (k & 3))) & 0xffu);
 }
-__device__ __forceinline__ int leg_cap(const Model &M, int lb) { return lb == 0 ? M.ccap[0] : lb == 1 ? M.ccap[1] : M.ccap[5]; }
-__device__ __forceinline__ int leg_slots(const Model &M) { return M.ccap[0] + M.ccap[1] + M.ccap[5]; }
 
-/* the torso's pose / velocity after lane T's root joint, in all four lanes */
-template <int T> __device__ __forceinline__ void bcast_pose(Body &B)
+/* contact slots of one lane: the torso's range, then its legs' femur
+ * ranges, then their tibia ranges (every leg's femur and tibia carry the
+ * same spheres: ranges of ccap[1] and ccap[5]) */
+template <int K> __device__ __forceinline__ int leg_slot(const Model &M, int kind, int l)
 {
-    qbcast3<T>(B.x);
-    B.q.w = qbcast<T>(B.q.w); B.q.x = qbcast<T>(B.q.x); B.q.y = qbcast<T>(B.q.y); B.q.z = qbcast<T>(B.q.z);
+    return kind == 0 ? 0 : kind == 1 ? M.ccap[0] + l * M.ccap[1] : M.ccap[0] + K * M.ccap[1] + l * M.ccap[5];
+}
+__device__ __forceinline__ int leg_cap(const Model &M, int kind) { return kind == 0 ? M.ccap[0] : kind == 1 ? M.ccap[1] : M.ccap[5]; }
+
+/* the torso's pose / velocity after lane T's root joints, in the group */
+template <int P, int T> __device__ __forceinline__ void bcast_pose(Body &B)
+{
+    gbcast3<P, T>(B.x);
+    B.q.w = gbcast<P, T>(B.q.w); B.q.x = gbcast<P, T>(B.q.x); B.q.y = gbcast<P, T>(B.q.y); B.q.z = gbcast<P, T>(B.q.z);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            B.R.e[r][c] = qbcast<T>(B.R.e[r][c]);
+            B.R.e[r][c] = gbcast<P, T>(B.R.e[r][c]);
 }
-template <int T> __device__ __forceinline__ void bcast_vel(Body &B)
+template <int P, int T> __device__ __forceinline__ void bcast_vel(Body &B)
 {
-    qbcast3<T>(B.v);
-    qbcast3<T>(B.w);
+    gbcast3<P, T>(B.v);
+    gbcast3<P, T>(B.w);
 }
 
-/* One control step of the quad's world (lane q).  Anchors: global sphere
- * indices; Poses: local bodies 0 torso, 1 coxa, 2 femur, 3 tibia; the
- * contact store addresses this lane's own list.  Returns this lane's
+/* One control step of the group's world by lane q (legs q K .. q K + K - 1).
+ * Anchors: global sphere indices; Poses: local bodies 0 torso, 1 + 3 l + m;
+ * the contact store addresses this lane's own list.  Returns this lane's
  * contact-overflow bits (bit per global contact body, as the lane form). */
-template <class Anchors, class Poses, class Tile>
-__device__ __forceinline__ int leg_step(LegWorld &W, int q0, Anchors &A0, Poses &P0, const Model &M,
+template <int K, class Anchors, class Poses, class Tile>
+__device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Poses &P0, const Model &M,
                                         const TerrainRef &tr0, const ContactStore &cs0, const Tile &T0)
 {
+    constexpr int P = LEGS / K;
     int overflow = 0;
     bsync<KISO>(W.T, robot::body[0]);
-    bsync<KISO>(W.b[0], robot::body[1]);
-    bsync<KANI>(W.b[1], robot::body[2]);
-    bsync<KANI>(W.b[2], robot::body[3]);
-    W.touchp[0] = W.touchp[1] = 0u;
-    W.tacc[0] = W.tacc[1] = W.tacc[2] = 0.0f;
+    static_for<0, K>([&](auto l) {
+        bsync<KISO>(W.b[l][0], robot::body[1]);
+        bsync<KANI>(W.b[l][1], robot::body[2]);
+        bsync<KANI>(W.b[l][2], robot::body[3]);
+    });
+    static_for<0, (1 + 5 * K + 3) / 4>([&](auto w) { W.touchp[w] = 0u; });
+    static_for<0, 3 * K>([&](auto j) { W.tacc[j] = 0.0f; });
 
     for (int sub0 = 0; sub0 < M.substeps; ++sub0) {
-        /* the leg index, opaque per substep: the per-leg joint constants are
+        /* the lane index, opaque per substep: the per-leg joint constants are
          * selected where used instead of being held across the step */
         const int q = (int)launder((size_t)q0);
         auto &&A = A0.fresh();
@@ -165,53 +182,65 @@ __device__ __forceinline__ int leg_step(LegWorld &W, int q0, Anchors &A0, Poses 
         TerrainRef tr = tr0;
         tr.q = launder(tr0.q);
         P0.store(0, integrate<KISO>(W.T, robot::body[0], M.g, M.h));
-        P0.store(1, integrate<KISO>(W.b[0], robot::body[1], M.g, M.h));
-        P0.store(2, integrate<KANI>(W.b[1], robot::body[2], M.g, M.h));
-        P0.store(3, integrate<KANI>(W.b[2], robot::body[3], M.g, M.h));
-
-        /* joint positions: the four roots in turn, then mids, then distals */
-        static_for<0, LEGS>([&](auto t) {
-            if (q == t) {
-                ROBOT_JOINT(3 * t, M);
-                joint_positions<KISO, KISO>(&W.T, &robot::body[0], W.b[0], robot::body[1], J, W.target[0], M.hh,
-                                            W.tacc[0]);
-            }
-            bcast_pose<t>(W.T);
+        static_for<0, K>([&](auto l) {
+            P0.store(1 + 3 * l, integrate<KISO>(W.b[l][0], robot::body[1], M.g, M.h));
+            P0.store(2 + 3 * l, integrate<KANI>(W.b[l][1], robot::body[2], M.g, M.h));
+            P0.store(3 + 3 * l, integrate<KANI>(W.b[l][2], robot::body[3], M.g, M.h));
         });
-        {
-            const JointK J = leg_joint<1>(q, M);
-            joint_positions<KISO, KANI>(&W.b[0], &robot::body[1], W.b[1], robot::body[2], J, W.target[1], M.hh,
-                                        W.tacc[1]);
-        }
-        {
-            const JointK J = leg_joint<2>(q, M);
-            joint_positions<KANI, KANI>(&W.b[1], &robot::body[2], W.b[2], robot::body[3], J, W.target[2], M.hh,
-                                        W.tacc[2]);
-        }
 
-        /* spheres: the torso's (every lane), then the leg's five in order */
-        int ncount[3] = { 0, 0, 0 };
+        /* joint positions: the four roots in turn (leg t's in lane t / K),
+         * the torso broadcast when the next root is another lane's; then
+         * each leg's mid and distal joints */
+        static_for<0, LEGS>([&](auto t) {
+            constexpr int owner = t / K, l = t % K;
+            if (q == owner) {
+                ROBOT_JOINT(3 * t, M);
+                joint_positions<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, W.target[3 * l],
+                                            M.hh, W.tacc[3 * l]);
+            }
+            if constexpr (l == K - 1)
+                bcast_pose<P, owner>(W.T);
+        });
+        static_for<0, K>([&](auto l) {
+            {
+                const JointK J = leg_joint<1>(q * K + l, M);
+                joint_positions<KISO, KANI>(&W.b[l][0], &robot::body[1], W.b[l][1], robot::body[2], J,
+                                            W.target[3 * l + 1], M.hh, W.tacc[3 * l + 1]);
+            }
+            {
+                const JointK J = leg_joint<2>(q * K + l, M);
+                joint_positions<KANI, KANI>(&W.b[l][1], &robot::body[2], W.b[l][2], robot::body[3], J,
+                                            W.target[3 * l + 2], M.hh, W.tacc[3 * l + 2]);
+            }
+        });
+
+        /* spheres: the torso's (every lane), then each leg's five in order */
+        int ncount[1 + 2 * K];
+        static_for<0, 1 + 2 * K>([&](auto c) { ncount[c] = 0; });
         const auto cells = T0.cells(M.terr, tr);
-        static_for<0, 6>([&](auto k) {
-            constexpr int LB = k == 0 ? 0 : (k == 1 ? 1 : 2); /* 0 torso, 1 femur, 2 tibia (contact lists) */
-            constexpr int GS = k == 0 ? 0 : (int)k;           /* the sphere's constants: leg 0's */
+        static_for<0, 1 + 5 * K>([&](auto k) {
+            constexpr int l = k == 0 ? 0 : (k - 1) / 5;        /* local leg */
+            constexpr int m = k == 0 ? 0 : (k - 1) % 5;        /* the leg's sphere */
+            constexpr int KIND = k == 0 ? 0 : (m == 0 ? 1 : 2); /* 0 torso, 1 femur, 2 tibia */
+            constexpr int NC = k == 0 ? 0 : (KIND == 1 ? 1 + l : 1 + K + l);
+            constexpr int GS = k == 0 ? 0 : 1 + m;              /* the sphere's constants: leg 0's */
             int anch = (W.anchored >> k) & 1u;
             int hit = 0;
-            auto rec = [&](V3 l, V3 n) {
-                if (ncount[LB] < leg_cap(M, LB))
-                    cs.put(leg_slot(M, LB) + ncount[LB], l, n);
+            auto rec = [&](V3 lp, V3 n) {
+                if (ncount[NC] < leg_cap(M, KIND))
+                    cs.put(leg_slot<K>(M, KIND, l) + ncount[NC], lp, n);
                 else
-                    overflow |= 1 << (LB == 0 ? 0 : LB == 1 ? 1 + q0 : 5 + q0);
-                ncount[LB]++;
+                    overflow |= 1 << (KIND == 0 ? 0 : KIND == 1 ? 1 + q0 * K + l : 5 + q0 * K + l);
+                ncount[NC]++;
             };
-            auto an = A.at(leg_sphere(q, k));
+            auto an = A.at(k == 0 ? 0 : 5 * (q * K + l) + 1 + m);
             if constexpr (k == 0) {
                 auto pose0 = [&]() { return P0.load(0); };
                 collide<KISO, sphere_at(0)>(W.T, robot::body[0], robot::sphere[0].at, robot::sphere[0].rad, hit,
                                             anch, an, pose0, M.terr, cells, rec, M.mus, M.mud);
             } else {
-                auto pose0 = [&]() { return P0.load(LB + 1); };
-                collide<KANI, sphere_at(GS)>(W.b[LB], robot::body[LB + 1], robot::sphere[GS].at,
+                auto pose0 = [&]() { return P0.load(1 + 3 * l + KIND); };
+                collide<KANI, sphere_at(GS)>(W.b[l][KIND], robot::body[1 + KIND], robot::sphere[GS].at,
                                              robot::sphere[GS].rad, hit, anch, an, pose0, M.terr, cells, rec,
                                              M.mus, M.mud);
             }
@@ -220,57 +249,71 @@ __device__ __forceinline__ int leg_step(LegWorld &W, int q0, Anchors &A0, Poses 
         });
 
         velocities(W.T, P0.load(0), M.inv_h, M.two_h);
-        velocities(W.b[0], P0.load(1), M.inv_h, M.two_h);
-        velocities(W.b[1], P0.load(2), M.inv_h, M.two_h);
-        velocities(W.b[2], P0.load(3), M.inv_h, M.two_h);
+        static_for<0, K>([&](auto l) {
+            velocities(W.b[l][0], P0.load(1 + 3 * l), M.inv_h, M.two_h);
+            velocities(W.b[l][1], P0.load(2 + 3 * l), M.inv_h, M.two_h);
+            velocities(W.b[l][2], P0.load(3 + 3 * l), M.inv_h, M.two_h);
+        });
 
         static_for<0, LEGS>([&](auto t) {
-            if (q == t) {
+            constexpr int owner = t / K, l = t % K;
+            if (q == owner) {
                 ROBOT_JOINT(3 * t, M);
-                joint_velocities<KISO, KISO>(&W.T, &robot::body[0], W.b[0], robot::body[1], J, M.h, W.tacc[0]);
+                joint_velocities<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, M.h,
+                                             W.tacc[3 * l]);
             }
-            bcast_vel<t>(W.T);
+            if constexpr (l == K - 1)
+                bcast_vel<P, owner>(W.T);
         });
-        {
-            const JointK J = leg_joint<1>(q, M);
-            joint_velocities<KISO, KANI>(&W.b[0], &robot::body[1], W.b[1], robot::body[2], J, M.h, W.tacc[1]);
-        }
-        {
-            const JointK J = leg_joint<2>(q, M);
-            joint_velocities<KANI, KANI>(&W.b[1], &robot::body[2], W.b[2], robot::body[3], J, M.h, W.tacc[2]);
-        }
+        static_for<0, K>([&](auto l) {
+            {
+                const JointK J = leg_joint<1>(q * K + l, M);
+                joint_velocities<KISO, KANI>(&W.b[l][0], &robot::body[1], W.b[l][1], robot::body[2], J, M.h,
+                                             W.tacc[3 * l + 1]);
+            }
+            {
+                const JointK J = leg_joint<2>(q * K + l, M);
+                joint_velocities<KANI, KANI>(&W.b[l][1], &robot::body[2], W.b[l][2], robot::body[3], J, M.h,
+                                             W.tacc[3 * l + 2]);
+            }
+        });
 
         /* recorded contacts, per body in record order */
-        static_for<0, 3>([&](auto lb) {
-            int n = ncount[lb] < leg_cap(M, lb) ? ncount[lb] : leg_cap(M, lb);
+        static_for<0, 1 + 2 * K>([&](auto c) {
+            constexpr int KIND = c == 0 ? 0 : (c <= K ? 1 : 2);
+            constexpr int l = c == 0 ? 0 : (c <= K ? c - 1 : c - 1 - K);
+            int n = ncount[c] < leg_cap(M, KIND) ? ncount[c] : leg_cap(M, KIND);
             for (int k = 0; k < n; ++k) {
-                V3 l, nn;
-                cs.get(leg_slot(M, lb) + k, l, nn);
-                if constexpr (lb == 0)
-                    contact_velocity<KISO>(W.T, robot::body[0], l, nn);
+                V3 lp, nn;
+                cs.get(leg_slot<K>(M, KIND, l) + k, lp, nn);
+                if constexpr (KIND == 0)
+                    contact_velocity<KISO>(W.T, robot::body[0], lp, nn);
                 else
-                    contact_velocity<KANI>(W.b[lb], robot::body[lb + 1], l, nn);
+                    contact_velocity<KANI>(W.b[l][KIND], robot::body[1 + KIND], lp, nn);
             }
         });
     }
 
-    /* epilogue: the lane's three joints, time, stability of the world */
+    /* epilogue: the lane's joints, time, stability of the world */
     const int q = q0;
-    {
-        const JointK J0 = leg_joint<0>(q, M), J1 = leg_joint<1>(q, M), J2 = leg_joint<2>(q, M);
-        W.angle[0] = joint_angle<true>(&W.T, W.b[0], J0);
-        W.rate[0] = joint_rate<true>(&W.T, W.b[0], J0);
-        W.angle[1] = joint_angle<true>(&W.b[0], W.b[1], J1);
-        W.rate[1] = joint_rate<true>(&W.b[0], W.b[1], J1);
-        W.angle[2] = joint_angle<true>(&W.b[1], W.b[2], J2);
-        W.rate[2] = joint_rate<true>(&W.b[1], W.b[2], J2);
-        W.torque[0] = W.tacc[0] / (float)M.substeps;
-        W.torque[1] = W.tacc[1] / (float)M.substeps;
-        W.torque[2] = W.tacc[2] / (float)M.substeps;
-    }
+    static_for<0, K>([&](auto l) {
+        const JointK J0 = leg_joint<0>(q * K + l, M), J1 = leg_joint<1>(q * K + l, M), J2 = leg_joint<2>(q * K + l, M);
+        W.angle[3 * l] = joint_angle<true>(&W.T, W.b[l][0], J0);
+        W.rate[3 * l] = joint_rate<true>(&W.T, W.b[l][0], J0);
+        W.angle[3 * l + 1] = joint_angle<true>(&W.b[l][0], W.b[l][1], J1);
+        W.rate[3 * l + 1] = joint_rate<true>(&W.b[l][0], W.b[l][1], J1);
+        W.angle[3 * l + 2] = joint_angle<true>(&W.b[l][1], W.b[l][2], J2);
+        W.rate[3 * l + 2] = joint_rate<true>(&W.b[l][1], W.b[l][2], J2);
+        W.torque[3 * l] = W.tacc[3 * l] / (float)M.substeps;
+        W.torque[3 * l + 1] = W.tacc[3 * l + 1] / (float)M.substeps;
+        W.torque[3 * l + 2] = W.tacc[3 * l + 2] / (float)M.substeps;
+    });
     W.time += M.dt;
-    int bad = body_unstable(W.T) | body_unstable(W.b[0]) | body_unstable(W.b[1]) | body_unstable(W.b[2]);
-    if (qor(bad))
+    int bad = body_unstable(W.T);
+    static_for<0, K>([&](auto l) {
+        bad |= body_unstable(W.b[l][0]) | body_unstable(W.b[l][1]) | body_unstable(W.b[l][2]);
+    });
+    if (gor<P>(bad))
         W.unstable = 1;
     return overflow;
 }

@@ -559,10 +559,10 @@ int arcx_sync(arc_sim *s)
 
 int arcx_set_form(arc_sim *s, int form)
 {
-    if (form < 0 || form > 2)
+    if (form < 0 || form > 3)
         return -1;
-    if (form == 2)
-        HIPCHK_RET(s, ensure_leg(s), "arcx_set_form: leg-form buffers", -1);
+    if (form >= 2)
+        HIPCHK_RET(s, ensure_leg(s), "arcx_set_form: group-form buffers", -1);
     s->form = form;
     return 0;
 }
@@ -578,9 +578,9 @@ int arcx_set_launch(arc_sim *s, int block, int grid_blocks)
 
 double arcx_bytes_per_world(const arc_sim *s)
 {
-    /* the world arrays plus the leg form's contact lists (held once that form is used) */
+    /* the world arrays plus the group forms' contact lists (held once one is used) */
     const Model &M = s->su.model;
-    const double leg = (double)(M.ccap[0] + M.ccap[1] + M.ccap[5]) * 6 * 4 * sizeof(float);
+    const double leg = (double)std::max(leg_world_slots(M, 1), leg_world_slots(M, 2)) * 6 * sizeof(float);
     return (double)dev_bytes_per_world(s->su.grid, M.cslots) + leg;
 }
 

@@ -248,9 +248,11 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
 }
 
 /* ------------------------------------------------------------------ */
-/* leg form: the four lanes of a quad step one world (leg_engine.h)     */
+/* group forms: the P = 4 / K lanes of a group step one world, K legs per
+ * lane (leg_engine.h) */
 
-__device__ __forceinline__ void load_leg(const DevWorlds &d, size_t i, int q, LegWorld &W)
+template <int K>
+__device__ __forceinline__ void load_leg(const DevWorlds &d, size_t i, int q, LegWorld<K> &W)
 {
     const size_t S = d.stride;
     auto body = [&](int b, Body &B) {
@@ -260,17 +262,23 @@ __device__ __forceinline__ void load_leg(const DevWorlds &d, size_t i, int q, Le
         B.w = v3(d.w[(b * 3 + 0) * S + i], d.w[(b * 3 + 1) * S + i], d.w[(b * 3 + 2) * S + i]);
     };
     body(0, W.T);
-    body(1 + 3 * q, W.b[0]);
-    body(2 + 3 * q, W.b[1]);
-    body(3 + 3 * q, W.b[2]);
+    static_for<0, K>([&](auto l) {
+        const int g = q * K + l;
+        body(1 + 3 * g, W.b[l][0]);
+        body(2 + 3 * g, W.b[l][1]);
+        body(3 + 3 * g, W.b[l][2]);
+    });
     const uint32_t a = d.anchored[i];
-    W.anchored = (a & 1u) | (((a >> (5 * q + 1)) & 31u) << 1);
+    constexpr uint32_t mask = (1u << (5 * K)) - 1u;
+    W.anchored = (a & 1u) | (((a >> (5 * q * K + 1)) & mask) << 1);
     W.unstable = d.unstable[i];
     W.time = d.time[i];
 }
 
-__device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, const LegWorld &W, int ov)
+template <int K>
+__device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, const LegWorld<K> &W, int ov)
 {
+    constexpr int P = LEGS / K;
     const size_t S = d.stride;
     auto body = [&](int b, const Body &B) {
         d.x[(b * 3 + 0) * S + i] = B.x.x; d.x[(b * 3 + 1) * S + i] = B.x.y; d.x[(b * 3 + 2) * S + i] = B.x.z;
@@ -279,19 +287,23 @@ __device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, c
         d.v[(b * 3 + 0) * S + i] = B.v.x; d.v[(b * 3 + 1) * S + i] = B.v.y; d.v[(b * 3 + 2) * S + i] = B.v.z;
         d.w[(b * 3 + 0) * S + i] = B.w.x; d.w[(b * 3 + 1) * S + i] = B.w.y; d.w[(b * 3 + 2) * S + i] = B.w.z;
     };
-    body(1 + 3 * q, W.b[0]);
-    body(2 + 3 * q, W.b[1]);
-    body(3 + 3 * q, W.b[2]);
-    for (int m = 0; m < 3; ++m) {
-        const int j = 3 * q + m;
-        d.jout[(0 * NJ + j) * S + i] = W.angle[m];
-        d.jout[(1 * NJ + j) * S + i] = W.rate[m];
-        d.jout[(2 * NJ + j) * S + i] = W.torque[m];
-    }
-    for (int k = 1; k < 6; ++k)
-        d.touch[(5 * q + k) * S + i] = (uint8_t)leg_touch(W, k);
-    const int anch = qor((int)(((W.anchored >> 1) & 31u) << (5 * q + 1)));
-    const int err = qor(ov);
+    static_for<0, K>([&](auto l) {
+        const int g = q * K + l;
+        body(1 + 3 * g, W.b[l][0]);
+        body(2 + 3 * g, W.b[l][1]);
+        body(3 + 3 * g, W.b[l][2]);
+        for (int m = 0; m < 3; ++m) {
+            const int j = 3 * g + m;
+            d.jout[(0 * NJ + j) * S + i] = W.angle[3 * l + m];
+            d.jout[(1 * NJ + j) * S + i] = W.rate[3 * l + m];
+            d.jout[(2 * NJ + j) * S + i] = W.torque[3 * l + m];
+        }
+        for (int m = 0; m < 5; ++m)
+            d.touch[(5 * g + 1 + m) * S + i] = (uint8_t)leg_touch(W, 1 + 5 * l + m);
+    });
+    constexpr uint32_t mask = (1u << (5 * K)) - 1u;
+    const int anch = gor<P>((int)(((W.anchored >> 1) & mask) << (5 * q * K + 1)));
+    const int err = gor<P>(ov);
     if (q == 0) {
         body(0, W.T);
         d.touch[i] = (uint8_t)leg_touch(W, 0);
@@ -303,15 +315,16 @@ __device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, c
     }
 }
 
-/* one control step of world i by lane q of its quad */
-template <int BLOCK, int TILE>
+/* one control step of world i by lane q of its group */
+template <int BLOCK, int TILE, int K>
 __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
                                                LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q)
 {
+    constexpr int P = LEGS / K;
     const size_t S = d.stride;
-    LegWorld W;
-    for (int m = 0; m < 3; ++m) {
-        const int j = 3 * q + m;
+    LegWorld<K> W;
+    for (int m = 0; m < 3 * K; ++m) {
+        const int j = 3 * q * K + m;
         float tg = targets[(size_t)i * NJ + j];
         W.target[m] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
         d.target[j * S + i] = W.target[m];
@@ -324,31 +337,31 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
     tr.outside = d.outside[i];
     ContactStore cs;
     cs.base = d.lcbuf;
-    cs.stride = 4 * S;
-    cs.world = (size_t)i * 4 + q;
-    constexpr int WPB = BLOCK / 4;
+    cs.stride = P * S;
+    cs.world = (size_t)i * P + q;
+    constexpr int WPB = BLOCK / P;
     int ov;
     if constexpr (TILE > 0) {
-        /* the quad's four lanes write the same bytes: no ordering needed */
-        const LdsTile<WPB, TILE> tile = fill_tile<WPB, TILE>(s_tile + (threadIdx.x >> 2), d, M.terr, (size_t)i,
+        /* the group's lanes write the same bytes: no ordering needed */
+        const LdsTile<WPB, TILE> tile = fill_tile<WPB, TILE>(s_tile + threadIdx.x / P, d, M.terr, (size_t)i,
                                                              d.x[0 * S + i], d.x[1 * S + i]);
-        load_leg(d, i, q, W);
-        ov = leg_step(W, q, A, P0, M, tr, cs, tile);
+        load_leg<K>(d, i, q, W);
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile);
     } else {
-        load_leg(d, i, q, W);
-        ov = leg_step(W, q, A, P0, M, tr, cs, GridTile());
+        load_leg<K>(d, i, q, W);
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile());
     }
-    store_leg(d, opaque_index((size_t)i), q, W, ov);
+    store_leg<K>(d, opaque_index((size_t)i), q, W, ov);
 }
 
-template <int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_step_leg(DevWorlds d, const Model *__restrict__ m,
-                                                    const float *__restrict__ targets,
-                                                    const StepBatch *__restrict__ batch, int total,
-                                                    int *__restrict__ work)
+template <int BLOCK, int TILE, int K>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 2 : 1))) void k_step_leg(
+    DevWorlds d, const Model *__restrict__ m, const float *__restrict__ targets, const StepBatch *__restrict__ batch,
+    int total, int *__restrict__ work)
 {
-    __shared__ float s_pose[4 * 7 * BLOCK];
-    __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * (BLOCK / 4) : 1];
+    constexpr int P = LEGS / K;
+    __shared__ float s_pose[(1 + 3 * K) * 7 * BLOCK];
+    __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * (BLOCK / P) : 1];
     const Model &M = *m;
     LdsPoses<BLOCK> P0;
     P0.p = s_pose;
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
     const int n_ranges = batch ? batch->n_ranges : 0;
     if (batch)
         total = batch->total;
-    const int items = 4 * total;
+    const int items = P * total;
     const int lane = threadIdx.x & 63;
     auto next = [&](int base) {
         if (!work)
@@ -370,8 +383,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
     while (base < items) {
         const int t = base + lane;
         if (t < items) {
-            const int w = t >> 2;
-            step_world_leg<BLOCK, TILE>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, w) : w, t & 3);
+            const int w = t / P;
+            step_world_leg<BLOCK, TILE, K>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, w) : w,
+                                           t % P);
         }
         base = next(base);
     }
@@ -386,9 +400,10 @@ int step_resident_blocks_of(int tile, int leg)
         || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
     hipError_t e;
-    if (leg)
-        e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16>, BLOCK, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 0>, BLOCK, 0);
+    if (leg == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16, 1>, BLOCK, 0);
+    else if (leg == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16, 2>, BLOCK, 0);
     else
         e = tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 16>, BLOCK, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step<BLOCK, 0>, BLOCK, 0);
@@ -402,19 +417,22 @@ template <int BLOCK>
 hipError_t launch_step_block(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
                              int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s)
 {
+    const int per_world = leg == 1 ? 4 : leg == 2 ? 2 : 1;
     if (grid_blocks <= 0)
-        grid_blocks = ((leg ? 4 : 1) * total + BLOCK - 1) / BLOCK;
+        grid_blocks = (per_world * total + BLOCK - 1) / BLOCK;
     if (grid_blocks <= 0)
         return hipSuccess;
     if (leg) {
-        if (!d.lcbuf)
+        /* the group forms always hold the terrain tile (the autotuner never
+         * picked them without it) */
+        if (!d.lcbuf || tile != 16)
             return hipErrorInvalidValue;
-        if (tile == 16)
-            hipLaunchKernelGGL((k_step_leg<BLOCK, 16>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch,
-                               total, work);
-        else if (tile == 0)
-            hipLaunchKernelGGL((k_step_leg<BLOCK, 0>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets, batch,
-                               total, work);
+        if (leg == 1)
+            hipLaunchKernelGGL((k_step_leg<BLOCK, 16, 1>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets,
+                               batch, total, work);
+        else if (leg == 2)
+            hipLaunchKernelGGL((k_step_leg<BLOCK, 16, 2>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets,
+                               batch, total, work);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();

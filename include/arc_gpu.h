@@ -75,7 +75,8 @@ int arcx_task_episodes(const arc_sim *s, unsigned int *out);
 /* Form of the step kernel: 0 = one world per lane (default; all substeps
  * of a control step in one launch, world state in registers), 1 = the plain
  * form, one world per wavefront with the world staged in LDS, 2 = four lanes
- * per world (one leg each, torso replicated; shorter per-world latency). */
+ * per world (one leg each, torso replicated; shortest per-world latency),
+ * 3 = two lanes per world (two legs each). */
 int arcx_set_form(arc_sim *s, int form);
 
 /* Launch shape of the step kernel: threads per block (a multiple of 64) and

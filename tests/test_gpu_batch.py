@@ -158,17 +158,18 @@ def test_batch_rejects_bad_requests(native):
         g.step_batch(list(range(257)), [0] * 257, dev.ptr)  # too many requests
 
 
-@pytest.mark.parametrize("tile", [16, 0])
-def test_leg_form_matches_lane_form(native, tile):
-    """Four lanes per world (torso replicated, roots chained through DPP
-    broadcasts) against one world per lane: bit-identical states, flags,
-    contact counts and joint outputs over several steps."""
+@pytest.mark.parametrize("form,tile", [("leg", 16), ("leg", 0), ("pair", 16)])
+def test_leg_form_matches_lane_form(native, form, tile):
+    """Four lanes per world (one leg each) and two lanes per world (two legs
+    each), torso replicated, hip joints chained through DPP broadcasts,
+    against one world per lane: bit-identical states, flags, contact counts
+    and joint outputs over several steps."""
     n, steps = 200, 4
     ts = action_stream(n, steps, seed=9).astype(np.float32)
     dev = native.DeviceBuffer.from_array(ts)
     per = n * 12 * 4
     lane, leg = _sim(native, n), _sim(native, n)
-    leg.set_form("leg")
+    leg.set_form(form)
     leg.set_layout(tile)
     lane.set_layout(tile)
     for k in range(steps):
@@ -181,13 +182,13 @@ def test_leg_form_matches_lane_form(native, tile):
     assert all(leg.world_error(i) == 0 for i in range(n))
 
 
-@pytest.mark.parametrize("persistent", [False, True])
-def test_leg_form_batches_and_graphs(native, persistent):
+@pytest.mark.parametrize("form,persistent", [("leg", False), ("leg", True), ("pair", False), ("pair", True)])
+def test_leg_form_batches_and_graphs(native, form, persistent):
     n = 256
     batches = [([0, 64], [32, 100]), ([5], [251]), ([0, 128, 200], [10, 60, 56])]
     t = native.DeviceBuffer(n * 12 * 4)
     a, b = _sim(native, n), _sim(native, n)
-    a.set_form("leg")
+    a.set_form(form)
     a.set_persistent(persistent)
     for k, (starts, counts) in enumerate(batches):
         t.upload(action_stream(n, k + 1, seed=6)[k].astype(np.float32))
