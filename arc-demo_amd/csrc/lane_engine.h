@@ -58,13 +58,25 @@ template <int I, int N, class F> XD void static_for(F &&f)
 
 static_assert(robot::NBODY == NB && robot::NJOINT == NJ && robot::NSPHERE == NS, "robot topology");
 
+/* Every drive has the same gain and torque limit (csrc/arc_robot.c:180-195),
+ * so the three products with the substep length are the same for all joints:
+ * the kernels read joint 0's, three values instead of thirty-six. */
+constexpr bool drives_alike()
+{
+    for (int j = 1; j < NJ; ++j)
+        if (!(robot::joint[j].kp == robot::joint[0].kp && robot::joint[j].tmax == robot::joint[0].tmax))
+            return false;
+    return true;
+}
+static_assert(drives_alike(), "the kernels share joint 0's substep products among all drives");
+
 /* joint constants as the kernel sees them: geometry and gains compiled in
  * (robot::joint), the three products with the substep length from the Model */
 #define ROBOT_JOINT(ji, M)                            \
     JointK J = robot::joint[ji];                      \
-    J.soft = (M).joint[ji].soft;                      \
-    J.cap_pos = (M).joint[ji].cap_pos;                \
-    J.tmax_h = (M).joint[ji].tmax_h
+    J.soft = (M).joint[0].soft;                       \
+    J.cap_pos = (M).joint[0].cap_pos;                 \
+    J.tmax_h = (M).joint[0].tmax_h
 
 struct Model {
     BodyK body[NB];

@@ -104,9 +104,9 @@ __device__ __forceinline__ JointK leg_joint(int q, const Model &M)
     J.b1 = sel4(q, a.b1, b.b1, c.b1, d.b1);
     J.b2 = sel4(q, a.b2, b.b2, c.b2, d.b2);
     J.qrel0 = sel4(q, a.qrel0, b.qrel0, c.qrel0, d.qrel0);
-    J.soft = M.joint[M_].soft;
-    J.cap_pos = M.joint[M_].cap_pos;
-    J.tmax_h = M.joint[M_].tmax_h;
+    J.soft = M.joint[0].soft; /* the same for every drive (drives_alike) */
+    J.cap_pos = M.joint[0].cap_pos;
+    J.tmax_h = M.joint[0].tmax_h;
     return J;
 }
 

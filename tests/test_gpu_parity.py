@@ -196,3 +196,25 @@ def test_small_grids(native, grid, cell):
     err = np.abs(a - b) / (np.abs(b) + 1.0)
     print(f"grid {grid} cell {cell}: max rel err {float(err.max()):.3g}, bit-exact worlds {same(a, b).all(1).mean():.3f}")
     assert float(err.max()) <= REL
+
+
+@pytest.mark.parametrize("form", ["lane", "leg"])
+@pytest.mark.parametrize("dt,substeps,grid,cell", [(1.0 / 60.0, 10, 64, 0.4), (1.0 / 20.0, 32, 48, 0.5)])
+def test_other_timesteps_and_grids(native, form, dt, substeps, grid, cell):
+    """Step length, substep count, grid and cell other than the defaults: the
+    compiled-in robot constants do not depend on them, the Model does."""
+    n = 64
+    diff = P.difficulties(2, n)
+    g = native.Sim(n, dt=dt, substeps=substeps, grid=grid, cell=cell)
+    g.set_form(form)
+    o = P.Lib("o32").sim(n, dt, substeps, grid, cell)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, (ids + 500).astype(np.uint32), diff)
+    o.reset(ids, (ids + 500).astype(np.uint32), diff)
+    ts = action_stream(n, 3, seed=4)
+    for k in range(3):
+        g.step(ts[k])
+        o.step(ts[k])
+        a, b = _states(g, n), _states(o, n)
+        err = np.abs(a - b) / (np.abs(b) + 1.0)
+        assert float(err.max()) <= (REL if k == 0 else 1e-3), f"step {k + 1}: {float(err.max()):.3g}"
