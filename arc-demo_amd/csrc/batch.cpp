@@ -51,6 +51,8 @@ struct BatchState {
     } cache[GRAPHS];
     long long clock = 0;
     int captures = 0;
+    StepBatch last;          /* the table last uploaded */
+    size_t last_bytes = 0;
 };
 
 void batch_free(arc_sim *s)
@@ -302,7 +304,14 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
         h->r[0].start = h->r[0].count = h->r[0].prefix = 0;
     }
     const size_t bytes = offsetof(StepBatch, r) + sizeof(Range) * (size_t)std::max(m, 1);
-    HIPCHK_RET(s, hipMemcpyAsync(b->d_batch, h, bytes, hipMemcpyHostToDevice, s->stream), "arcx_step_batch table", -1);
+    /* the device table is rewritten only when the batch differs from the last
+     * one uploaded (stream order keeps the kernels reading the right one) */
+    if (bytes != b->last_bytes || memcmp(h, &b->last, bytes) != 0) {
+        HIPCHK_RET(s, hipMemcpyAsync(b->d_batch, h, bytes, hipMemcpyHostToDevice, s->stream), "arcx_step_batch table",
+                   -1);
+        memcpy(&b->last, h, bytes);
+        b->last_bytes = bytes;
+    }
     HIPCHK_RET(s, hipEventRecord(b->ev[slot], s->stream), "arcx_step_batch", -1);
     if (!graph) {
         if (total == 0 && !task)
