@@ -101,6 +101,8 @@ def _task_lib():
             ("arct_levels", _int, [_vp, _vp]),
             ("arct_set_levels", _int, [_vp, _vp]),
             ("arct_last_resets", _int, [_vp, _vp, _vp]),
+            ("arct_seed_numpy", _int, [_vp, _vp, _int]),
+            ("arct_numpy_state", _int, [_vp, _vp]),
         ]:
             fn = getattr(lib, name)
             fn.restype = res
@@ -193,8 +195,29 @@ def _pointer(actions) -> int:
     raise TypeError("actions: a device pointer, a torch tensor or a DeviceArray")
 
 
+def seed_words(seed: int | None) -> np.ndarray:
+    """numpy's SeedSequence entropy of `seed` as little-endian 32-bit words
+    (None: fresh OS entropy, as np.random.default_rng(None))"""
+    if seed is None:
+        seed = np.random.SeedSequence().entropy
+    seed = int(seed)
+    if seed < 0:
+        raise ValueError("seed must be non-negative")
+    w = []
+    while True:
+        w.append(seed & 0xFFFFFFFF)
+        seed >>= 32
+        if not seed:
+            return np.array(w, dtype=np.uint32)
+
+
 class DeviceTask:
-    def __init__(self, num_worlds: int, config: TaskConfig | None = None, seed: int = 0, form: str | None = None):
+    """rng="world": checkpoints and reset seeds from per-world streams seeded by
+    `seed`; rng="numpy": from np.random.default_rng(seed) in the reference's
+    order (the Python task's draws for the same seed)."""
+
+    def __init__(self, num_worlds: int, config: TaskConfig | None = None, seed: int | None = 0,
+                 form: str | None = None, rng: str = "world"):
         self.cfg = cfg = config or TaskConfig()
         self._lib = _task_lib()
         self.sim = Sim(num_worlds, cfg.dt, cfg.substeps, cfg.grid, cfg.cell)
@@ -206,9 +229,14 @@ class DeviceTask:
             if name in dict(_Config._fields_):
                 setattr(c, name, int(getattr(cfg, f.name)) if name in ("curriculum",) else getattr(cfg, f.name))
         off = np.ascontiguousarray(HEIGHT_OFFSETS, dtype=np.float64)
-        self._ptr = self._lib.arct_create(self.sim.ptr, ctypes.byref(c), off.ctypes.data, len(off), seed)
+        if rng not in ("world", "numpy"):
+            raise ValueError(f"rng: 'world' or 'numpy', not {rng!r}")
+        self._ptr = self._lib.arct_create(self.sim.ptr, ctypes.byref(c), off.ctypes.data, len(off),
+                                          (seed or 0) & 0xFFFFFFFF)
         if not self._ptr:
             raise RuntimeError("arct_create failed")
+        if rng == "numpy":
+            self.seed_numpy(seed)
         n, self.num_worlds, self.nj = num_worlds, num_worlds, 12
         self.obs_size = self._lib.arct_obs_size(self._ptr)
         buf = lambda k: self._lib.arct_buffer(self._ptr, k)
@@ -220,6 +248,22 @@ class DeviceTask:
         self.info = DeviceArray(self, buf(5), (3, n), "i32")  # cause, checkpoints, steps
         self.difficulty = DeviceArray(self, buf(6), (n,), "f64")
         self.actions = DeviceArray(self, buf(7), (n, 12), "f32")
+
+    def seed_numpy(self, seed: int | None) -> None:
+        """from now on draw from np.random.default_rng(seed), as the reference's
+        ArcTask.rng (arc/task.py:93, arc/env.py:93-95)"""
+        w = seed_words(seed)
+        if self._lib.arct_seed_numpy(self._ptr, w.ctypes.data, len(w)):
+            self.sim._check()
+
+    def numpy_state(self) -> dict:
+        """the generator's state in the layout of numpy's bit_generator.state"""
+        out = np.zeros(6, dtype=np.uint64)
+        if self._lib.arct_numpy_state(self._ptr, out.ctypes.data):
+            self.sim._check()
+        return {"bit_generator": "PCG64",
+                "state": {"state": int(out[0]) << 64 | int(out[1]), "inc": int(out[2]) << 64 | int(out[3])},
+                "has_uint32": int(out[4]), "uinteger": int(out[5])}
 
     def sync(self) -> None:
         self._lib.arct_sync(self._ptr)

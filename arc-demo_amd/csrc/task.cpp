@@ -4,6 +4,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 #include <vector>
@@ -18,13 +19,13 @@ struct arc_task {
     TaskCfg cfg{};
     TaskDev td{};
     TaskOut out{};
-    void *mem = nullptr;
+    void *mem = nullptr, *np_mem = nullptr;
     float *obs = nullptr, *term = nullptr, *actions = nullptr, *targets = nullptr;
     double *offsets = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const float *graph_actions = nullptr;
-    int graph_key[5] = { 0, 0, 0, 0, 0 };
+    int graph_key[6] = { 0, 0, 0, 0, 0, 0 };
 };
 
 extern "C" {
@@ -73,6 +74,8 @@ void arct_destroy(arc_task *t)
         (void)hipGraphDestroy(t->graph);
     if (t->mem)
         (void)hipFree(t->mem);
+    if (t->np_mem)
+        (void)hipFree(t->np_mem);
     delete t;
 }
 
@@ -171,6 +174,20 @@ arc_task *arct_create(arc_sim *s, const arct_config *c, const double *offsets2k,
     uint8_t *bp = (uint8_t *)fp;
     t->out.terminated = bp; bp += N;
     t->out.truncated = bp; bp += N;
+    /* the numpy-generator mode's generator and lists */
+    char *q = nullptr;
+    if (hipMalloc((void **)&q, sizeof(NpStream) + 4 * N * sizeof(int)) != hipSuccess) {
+        fprintf(stderr, "libarcsim: arct_create: device allocation failed\n");
+        arct_destroy(t);
+        return nullptr;
+    }
+    t->np_mem = q;
+    td.np = (NpStream *)q;
+    int *lp = (int *)(q + sizeof(NpStream));
+    td.flag = lp; lp += N;
+    td.hits = lp; lp += N;
+    td.wa = lp; lp += N;
+    td.wb = lp; lp += N;
     hipError_t e = hipSuccess;
     if (k)
         e = hipMemcpyAsync(t->offsets, offsets2k, 2 * (size_t)k * sizeof(double), hipMemcpyHostToDevice, s->stream);
@@ -207,12 +224,16 @@ void *arct_buffer(arc_task *t, int which)
 static hipError_t reset_listed(arc_task *t)
 {
     arc_sim *s = t->sim;
+    const bool np = t->cfg.np_stream != 0;
     hipError_t e;
-    if ((e = task_launch_reset_args(s->dev, t->td, s->stream)) != hipSuccess
+    if ((e = np ? task_launch_np_seeds(t->td, s->stream) : task_launch_reset_args(s->dev, t->td, s->stream))
+            != hipSuccess
         || (e = launch_reset(s->dev, s->d_model, s->d_reset, t->td.list, t->td.seeds, nullptr, t->td.diff, 0,
                              t->td.count, s->reset_blocks, s->stream)) != hipSuccess
         || (e = task_launch_after_reset(s->dev, s->d_model, t->td, t->cfg, s->stream)) != hipSuccess)
         return e;
+    if (np)
+        return task_launch_np_place(s->dev, s->d_model, t->td, t->cfg, s->stream);
     return hipSuccess;
 }
 
@@ -235,6 +256,7 @@ static hipError_t enqueue_step(arc_task *t, const float *actions)
         || (e = task_launch_actions(s->dev, t->td, t->cfg, actions, t->targets, s->stream)) != hipSuccess
         || (e = launch_form(s, t->targets)) != hipSuccess
         || (e = task_launch_post(s->dev, s->d_model, t->td, t->cfg, t->out, s->stream)) != hipSuccess
+        || (t->cfg.np_stream && (e = task_launch_np_hits(s->dev, s->d_model, t->td, t->cfg, s->stream)) != hipSuccess)
         || (e = task_launch_observe(s->dev, s->d_model, t->td, t->cfg, t->offsets, t->k, t->obs, false, s->stream))
                != hipSuccess
         || (e = task_launch_copy_listed(s->dev, t->td, t->obs, t->term, t->size, s->stream)) != hipSuccess
@@ -251,7 +273,7 @@ int arct_step(arc_task *t, const float *actions_dev, int graph)
         HIPCHK_RET(s, enqueue_step(t, a), "arct_step", -1);
         return 0;
     }
-    const int key[5] = { s->form, s->block, s->tile, s->persistent, s->grid_blocks };
+    const int key[6] = { s->form, s->block, s->tile, s->persistent, s->grid_blocks, t->cfg.np_stream };
     if (!t->exec || t->graph_actions != a || memcmp(key, t->graph_key, sizeof(key))) {
         /* buffers the launch may allocate on first use are made before capture */
         HIPCHK_RET(s, prepare_launch(s), "arct_step prepare", -1);
@@ -297,6 +319,42 @@ int arct_targets(const arc_task *ct, double *out3)
         out3[3 * i + 1] = buf[S + i];
         out3[3 * i + 2] = buf[2 * S + i];
     }
+    return 0;
+}
+
+int arct_seed_numpy(arc_task *t, const unsigned int *words, int n_words)
+{
+    arc_sim *s = t->sim;
+    if (n_words < 0 || (n_words > 0 && !words)) {
+        set_error(s, -1, "arct_seed_numpy: bad seed words", hipSuccess);
+        return -1;
+    }
+    NpStream g;
+    np_seed(g, words, n_words);
+    HIPCHK_RET(s, hipMemcpyAsync(t->td.np, &g, sizeof(g), hipMemcpyHostToDevice, s->stream), "arct_seed_numpy", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arct_seed_numpy", -1);
+    t->cfg.np_stream = 1;
+    return 0;
+}
+
+int arct_numpy_state(const arc_task *ct, unsigned long long *out6)
+{
+    arc_task *t = const_cast<arc_task *>(ct);
+    arc_sim *s = t->sim;
+    if (!t->cfg.np_stream) {
+        set_error(s, -1, "arct_numpy_state: the task does not draw from a numpy generator", hipSuccess);
+        return -1;
+    }
+    NpStream g;
+    HIPCHK_RET(s, hipMemcpyAsync(&g, t->td.np, offsetof(NpStream, pow2), hipMemcpyDeviceToHost, s->stream),
+               "arct_numpy_state", -1);
+    HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arct_numpy_state", -1);
+    out6[0] = (unsigned long long)(g.state >> 64);
+    out6[1] = (unsigned long long)g.state;
+    out6[2] = (unsigned long long)(g.inc >> 64);
+    out6[3] = (unsigned long long)g.inc;
+    out6[4] = g.has_uint32;
+    out6[5] = g.uinteger;
     return 0;
 }
 

@@ -7,8 +7,11 @@
  * The task's arithmetic is binary64 on the binary32 engine state, as the
  * reference task computes on float64 copies of the sim's outputs; its
  * transcendental functions are the device's binary64 ones.  Checkpoint
- * placement and reset seeds draw from a per-world xorshift128 stream of the
- * engine's generator instead of the task's numpy generator.
+ * placement and reset seeds draw either from a per-world xorshift128 stream
+ * of the engine's generator, or (numpy mode) from one numpy default_rng
+ * stream in the reference's order: the hits of a step in world order, then
+ * the seeds of the ended worlds, then their first checkpoints
+ * (arc/task.py:112-150,232-234, arc/env.py:99-117).
  */
 #include "task_state.h"
 
@@ -75,7 +78,40 @@ __device__ double task_uniform(TaskDev &td, size_t S, size_t i, double lo, doubl
     return lo + (hi - lo) * ((double)u * 2.3283064365386963e-10);
 }
 
-/* the next checkpoint, 3-6 m away inside the terrain, arc/task.py:127-150 */
+/* world i's position and rotation in binary64, as observe() reports them */
+__device__ void task_pose(const DevWorlds &d, size_t i, double pos[3], double R[3][3])
+{
+    const size_t S = d.stride;
+    Q4 q = q4(d.q[0 * S + i], d.q[1 * S + i], d.q[2 * S + i], d.q[3 * S + i]);
+    M3 m;
+    m_from_q(m, q);
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = d.x[a * S + i];
+        for (int c = 0; c < 3; ++c)
+            R[a][c] = m.e[a][c];
+    }
+}
+
+/* a placed checkpoint and what follows from it, arc/task.py:145-150 */
+__device__ void set_target(const DevWorlds &d, const Model &M, TaskDev td, const TaskCfg &c, size_t i,
+                           const double pos[3], const double R[3][3], double tx, double ty)
+{
+    const size_t S = d.stride;
+    td.target[0 * S + i] = tx;
+    td.target[1 * S + i] = ty;
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
+    tr.outside = d.outside[i];
+    td.target_z[i] = terrain_height(M.terr, tr, (float)tx, (float)ty);
+    const double tg[2] = { tx, ty };
+    td.prev_dist[i] = hypot(tx - pos[0], ty - pos[1]);
+    const double g = gaze_error(R, pos, tg);
+    td.prev_gaze[i] = g;
+    td.in_view[i] = g < c.fov_rad;
+}
+
+/* the next checkpoint, 3-6 m away inside the terrain, arc/task.py:127-150,
+ * from world i's own stream */
 __device__ void new_target(const DevWorlds &d, const Model &M, TaskDev td, const TaskCfg &c, size_t i,
                            const TaskView &t)
 {
@@ -97,17 +133,7 @@ __device__ void new_target(const DevWorlds &d, const Model &M, TaskDev td, const
         tx = 0.5 * t.pos[0];
         ty = 0.5 * t.pos[1];
     }
-    td.target[0 * S + i] = tx;
-    td.target[1 * S + i] = ty;
-    TerrainRef tr = terrain_ref(d, i);
-    tr.base = d.tbase[i];
-    tr.outside = d.outside[i];
-    td.target_z[i] = terrain_height(M.terr, tr, (float)tx, (float)ty);
-    const double tg[2] = { tx, ty };
-    td.prev_dist[i] = hypot(tx - t.pos[0], ty - t.pos[1]);
-    const double g = gaze_error(t.R, t.pos, tg);
-    td.prev_gaze[i] = g;
-    td.in_view[i] = g < c.fov_rad;
+    set_target(d, M, td, c, i, t.pos, t.R, tx, ty);
 }
 
 /* drive targets from the actions: default + scale * clip(a, -1, 1) */
@@ -195,7 +221,7 @@ __global__ void k_task_post(DevWorlds d, const Model *__restrict__ m, TaskDev td
         td.prev_action[j * S + i] = td.action[j * S + i];
     const int cps = td.checkpoints[i] + (reached ? 1 : 0);
     td.checkpoints[i] = cps;
-    if (reached && !fell)
+    if (reached && !fell && !c.np_stream)
         new_target(d, M, td, c, i, t);
 
     const bool truncated = steps >= c.max_steps && !fell;
@@ -214,9 +240,13 @@ __global__ void k_task_post(DevWorlds d, const Model *__restrict__ m, TaskDev td
             lvl = rint(lvl * 1e6) / 1e6;
             td.level[i] = lvl < 0.0 ? 0.0 : (lvl > c.max_difficulty ? c.max_difficulty : lvl);
         }
-        const int slot = atomicAdd(td.count, 1);
-        td.list[slot] = (int)i;
+        if (!c.np_stream) {
+            const int slot = atomicAdd(td.count, 1);
+            td.list[slot] = (int)i;
+        }
     }
+    if (c.np_stream) /* the ordered lists are built by k_np_hits */
+        td.flag[i] = (reached && !fell ? 1 : 0) | (fell || truncated ? 2 : 0);
 }
 
 /* seeds and difficulties of the listed worlds' resets; each world draws its
@@ -252,6 +282,8 @@ __global__ void k_task_after_reset(DevWorlds d, const Model *__restrict__ m, Tas
     td.torso_steps[i] = 0;
     td.steps[i] = 0;
     td.checkpoints[i] = 0;
+    if (c.np_stream) /* placed by k_np_place */
+        return;
     TaskView t;
     task_view(d, *m, (size_t)i, t);
     new_target(d, *m, td, c, (size_t)i, t);
@@ -367,6 +399,165 @@ __global__ void k_task_copy_listed(TaskDev td, const float *__restrict__ obs, fl
         dst[i * size + a] = obs[i * size + a];
 }
 
+/* ---- numpy-generator mode: one workgroup consumes the stream in order ---- */
+
+enum { NPW = 1024 };
+
+/* rank of this thread among the workgroup's threads with `flag` set, in
+ * thread order; the count of all of them in `total` */
+__device__ int wg_rank(bool flag, int *lds, int &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0)
+        lds[wave] = __popcll(m);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < NPW / 64; ++w) {
+        const int c = lds[w];
+        before += w < wave ? c : 0;
+        all += c;
+    }
+    __syncthreads();
+    total = all;
+    return before + in_wave;
+}
+
+/* checkpoints of the worlds list[0..m), arc/task.py:127-150 with the
+ * generator at state s: each round draws an angle for every pending world
+ * in list order, then a distance for every pending world; a candidate inside
+ * the border is taken, the others stay pending (in order) for the next of at
+ * most 20 rounds; the rest go halfway towards the centre.  Called by the
+ * whole workgroup; s is the generator state after the draws. */
+__device__ void np_place(const DevWorlds &d, const Model &M, TaskDev td, const TaskCfg &c, const int *list, int m,
+                         u128 &s, int *lds)
+{
+    const NpStream &g = *td.np;
+    const double limit = c.half_extent - c.border;
+    const double pi = 3.141592653589793;
+    const double dlo = c.checkpoint_min_dist, dspan = c.checkpoint_max_dist - c.checkpoint_min_dist;
+    const int *src = list;
+    int *dst = td.wa;
+    int K = m;
+    for (int round = 0; round < 20 && K > 0; ++round) {
+        int next = 0;
+        for (int k0 = 0; k0 < K; k0 += NPW) {
+            const int k = k0 + (int)threadIdx.x;
+            bool pending = false;
+            int i = 0;
+            if (k < K) {
+                i = src[k];
+                double pos[3], R[3][3];
+                task_pose(d, (size_t)i, pos, R);
+                const double ang = -pi + (pi - -pi) * pcg_unit(np_draw(g, s, (uint64_t)k));
+                const double dist = dlo + dspan * pcg_unit(np_draw(g, s, (uint64_t)K + (uint64_t)k));
+                const double cx = pos[0] + dist * cos(ang), cy = pos[1] + dist * sin(ang);
+                if (fabs(cx) < limit && fabs(cy) < limit)
+                    set_target(d, M, td, c, (size_t)i, pos, R, cx, cy);
+                else
+                    pending = true;
+            }
+            int total;
+            const int r = wg_rank(pending, lds, total);
+            if (pending)
+                dst[next + r] = i;
+            next += total;
+        }
+        s = np_advance(g, s, 2 * (uint64_t)K);
+        K = next;
+        src = dst;
+        dst = dst == td.wa ? td.wb : td.wa;
+        __syncthreads(); /* the pending list is complete before the next round reads it */
+    }
+    for (int k = (int)threadIdx.x; k < K; k += NPW) {
+        const int i = src[k];
+        double pos[3], R[3][3];
+        task_pose(d, (size_t)i, pos, R);
+        set_target(d, M, td, c, (size_t)i, pos, R, 0.5 * pos[0], 0.5 * pos[1]);
+    }
+}
+
+/* after k_task_post: the hit and ended worlds in world order (np.flatnonzero),
+ * the hits' new checkpoints (arc/task.py:232-234), the ended worlds as the
+ * reset list */
+__global__ __launch_bounds__(NPW) void k_np_hits(DevWorlds d, const Model *__restrict__ m, TaskDev td, TaskCfg c)
+{
+    __shared__ int lds[NPW / 64];
+    const int N = d.n;
+    int nh = 0, ne = 0;
+    for (int i0 = 0; i0 < N; i0 += NPW) {
+        const int i = i0 + (int)threadIdx.x;
+        const int f = i < N ? td.flag[i] : 0;
+        int th, te;
+        const int rh = wg_rank((f & 1) != 0, lds, th);
+        const int re = wg_rank((f & 2) != 0, lds, te);
+        if (f & 1)
+            td.hits[nh + rh] = i;
+        if (f & 2)
+            td.list[ne + re] = i;
+        nh += th;
+        ne += te;
+    }
+    if (threadIdx.x == 0)
+        *td.count = ne;
+    __syncthreads();
+    u128 s = td.np->state;
+    np_place(d, *m, td, c, td.hits, nh, s, lds);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        td.np->state = s;
+}
+
+/* seeds of the listed worlds' resets, rng.integers(0, 2**32, n, uint32)
+ * (arc/task.py:117): 32-bit halves of 64-bit outputs, low first, starting with
+ * a half the generator kept from an earlier call; and their difficulties */
+__global__ __launch_bounds__(NPW) void k_np_seeds(TaskDev td)
+{
+    NpStream &g = *td.np;
+    const int n = *td.count;
+    const u128 s = g.state;
+    const int has = (int)g.has_uint32;
+    const uint32_t kept = g.uinteger;
+    for (int k = (int)threadIdx.x; k < n; k += NPW) {
+        uint32_t v;
+        if (k < has) {
+            v = kept;
+        } else {
+            const int t = k - has;
+            const uint64_t x = np_draw(g, s, (uint64_t)(t >> 1));
+            v = (t & 1) ? (uint32_t)(x >> 32) : (uint32_t)x;
+        }
+        td.seeds[k] = v;
+        const int i = td.list[k];
+        td.difficulty[i] = td.level[i];
+        td.diff[k] = (float)td.level[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && n > 0) {
+        const int r = n - has; /* halves taken from new outputs */
+        if (r == 0) {
+            g.has_uint32 = 0u;
+        } else {
+            const uint64_t used = (uint64_t)(r + 1) / 2;
+            g.uinteger = (uint32_t)(np_draw(g, s, used - 1) >> 32);
+            g.state = np_advance(g, s, used);
+            g.has_uint32 = (uint32_t)(r & 1);
+        }
+    }
+}
+
+/* first checkpoints of the listed (just reset) worlds, arc/task.py:125 */
+__global__ __launch_bounds__(NPW) void k_np_place(DevWorlds d, const Model *__restrict__ m, TaskDev td, TaskCfg c)
+{
+    __shared__ int lds[NPW / 64];
+    u128 s = td.np->state;
+    np_place(d, *m, td, c, td.list, *td.count, s, lds);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        td.np->state = s;
+}
+
 static unsigned blocks_for(int n) { return (unsigned)((n > 0 ? n : 1) + 255) / 256; }
 
 hipError_t task_launch_init(const DevWorlds &d, const TaskDev &td, uint32_t seed, double level, hipStream_t s)
@@ -406,6 +597,23 @@ hipError_t task_launch_after_reset(const DevWorlds &d, const Model *m, const Tas
                                    hipStream_t s)
 {
     hipLaunchKernelGGL(k_task_after_reset, dim3(blocks_for(d.n)), dim3(256), 0, s, d, m, td, c);
+    return hipGetLastError();
+}
+hipError_t task_launch_np_hits(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_np_hits, dim3(1), dim3(NPW), 0, s, d, m, td, c);
+    return hipGetLastError();
+}
+hipError_t task_launch_np_seeds(const TaskDev &td, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_np_seeds, dim3(1), dim3(NPW), 0, s, td);
+    return hipGetLastError();
+}
+hipError_t task_launch_np_place(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_np_place, dim3(1), dim3(NPW), 0, s, d, m, td, c);
     return hipGetLastError();
 }
 hipError_t task_launch_observe(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,

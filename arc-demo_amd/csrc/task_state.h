@@ -4,6 +4,7 @@
  */
 #pragma once
 #include "device_state.h"
+#include "pcg64.h"
 
 namespace xpbd {
 
@@ -22,6 +23,7 @@ struct TaskCfg {
     int level_up_checkpoints, level_down_steps;
     double half_extent, stand;
     double def[NJ];
+    int np_stream; /* 1: draws from one numpy generator in the reference's order */
 };
 
 /* per-world task state, [field][stride] */
@@ -39,6 +41,10 @@ struct TaskDev {
     uint32_t *seeds;
     float *diff;
     int *count;
+    /* numpy-generator mode: per-world hit / ended flags, the ordered hit list,
+     * the pending lists of checkpoint placement, the generator */
+    int *flag, *hits, *wa, *wb;
+    NpStream *np;
 };
 
 /* per-step outputs */
@@ -60,6 +66,11 @@ hipError_t task_launch_copy_listed(const DevWorlds &d, const TaskDev &td, const 
 hipError_t task_launch_reset_args(const DevWorlds &d, const TaskDev &td, hipStream_t s);
 hipError_t task_launch_after_reset(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
                                    hipStream_t s);
+hipError_t task_launch_np_hits(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
+                               hipStream_t s);
+hipError_t task_launch_np_seeds(const TaskDev &td, hipStream_t s);
+hipError_t task_launch_np_place(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
+                                hipStream_t s);
 hipError_t task_launch_observe(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
                                const double *offsets, int k_off, float *obs, bool listed, hipStream_t s);
 

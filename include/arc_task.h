@@ -8,10 +8,12 @@
  * ended worlds and re-observes them (arc/env.py:99-117).  Here one call does
  * all of it on the GPU, with the actions read from and the observations
  * written to device memory, so a policy on the same GPU exchanges them
- * without host copies.  Differences: checkpoint placement and reset seeds
- * come from a per-world stream of the engine's xorshift128 generator (the
- * reference draws them from one numpy PCG64 generator); the rules, ranges and
- * order of the draws' uses are the reference's.
+ * without host copies.  Checkpoint placement and reset seeds come either
+ * from a per-world stream of the engine's xorshift128 generator (the
+ * default: independent of which worlds end together), or, after
+ * arct_seed_numpy, from one stream of numpy's default generator consumed in
+ * the reference's order, so that they equal the Python task's with the same
+ * seed (arc/task.py:93,117,136-137).
  */
 #ifndef ARC_TASK_H
 #define ARC_TASK_H
@@ -74,6 +76,21 @@ int arct_reset_all(arc_task *t);
  * configuration change).  Asynchronous; arct_sync waits. */
 int arct_step(arc_task *t, const float *actions_dev, int graph);
 int arct_sync(arc_task *t);
+
+/* Draw checkpoints and reset seeds from np.random.default_rng(seed), as the
+ * reference's ArcTask (arc/task.py:93) and ArcVecEnv.reset (arc/env.py:93-95)
+ * seed it; `words` is the seed as little-endian 32-bit words (numpy's
+ * SeedSequence entropy; one word 0 for n_words = 0).  The draws follow the
+ * reference's order: per step, new checkpoints of the worlds that reached
+ * theirs (in world order: rounds of angles then distances, arc/task.py:127-150),
+ * then the seeds of the ended worlds (rng.integers(0, 2**32, n, uint32),
+ * arc/task.py:117), then their first checkpoints.  Takes effect from the next
+ * arct_reset_all / arct_step. */
+int arct_seed_numpy(arc_task *t, const unsigned int *words, int n_words);
+/* the generator's state as numpy's bit_generator.state holds it: state (hi,
+ * lo 64-bit words), inc (hi, lo), has_uint32, uinteger; -1 unless seeded by
+ * arct_seed_numpy */
+int arct_numpy_state(const arc_task *t, unsigned long long *out6);
 
 /* curriculum levels of every world (N doubles, host): the difficulty of each
  * world's next episode (with curriculum off, its fixed difficulty) */

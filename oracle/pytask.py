@@ -4,11 +4,12 @@ order (arc/env.py:99-117), over the oracle sim (pyoracle.Sim).
 
 Test infrastructure: only tests/ use it, as the checker of the device task
 (include/arc_task.h).  The reference draws reset seeds and checkpoint
-positions from one numpy PCG64 generator; the device draws them from its own
-per-world streams, so this oracle takes them from the caller (the test reads
-them back from the device) and restates everything else: observations,
-rewards, terminations and their precedence, the checkpoint rule's
-consequences (terrain height, distance, gaze, view), the curriculum.
+positions from one numpy generator (arc/task.py:93,117,136-137).  Given a
+numpy Generator (`rng`), this oracle draws them as the reference does, for the
+device's numpy mode; without one it takes them from the caller (the device's
+per-world streams, read back by the test).  It restates everything else:
+observations, rewards, terminations and their precedence, the checkpoint
+rule's consequences (terrain height, distance, gaze, view), the curriculum.
 """
 
 from __future__ import annotations
@@ -27,8 +28,8 @@ def gaze_error(rot: np.ndarray, pos: np.ndarray, target: np.ndarray) -> np.ndarr
 
 
 class OracleTask:
-    def __init__(self, sim, cfg, offsets: np.ndarray, grid: int, cell: float):
-        self.sim, self.cfg, self.offsets = sim, cfg, offsets
+    def __init__(self, sim, cfg, offsets: np.ndarray, grid: int, cell: float, rng=None):
+        self.sim, self.cfg, self.offsets, self.rng = sim, cfg, offsets, rng
         n = sim.n
         self.n = n
         d, _, _, misc = sim.robot_info()
@@ -59,11 +60,35 @@ class OracleTask:
         self.prev_gaze[ids] = gaze_error(self.rot[ids], pos, target)
         self.in_view[ids] = self.prev_gaze[ids] < np.radians(self.cfg.fov)
 
-    def reset(self, ids, seeds, targets):
-        """arc/task.py:108-125 with the seeds and first checkpoints given"""
+    def _draw_targets(self, ids):
+        """arc/task.py:127-144 on self.rng: rounds over the pending worlds, each
+        drawing all angles, then all distances; inside-the-border candidates
+        are taken; after 20 rounds the rest go halfway to the centre"""
+        cfg = self.cfg
+        pos = self.root[ids, :2]
+        limit = self.half_extent - cfg.border
+        out = 0.5 * pos
+        pending = np.arange(len(ids))
+        for _ in range(20):
+            if len(pending) == 0:
+                break
+            ang = self.rng.uniform(-np.pi, np.pi, len(pending))
+            dist = self.rng.uniform(cfg.checkpoint_min_dist, cfg.checkpoint_max_dist, len(pending))
+            cand = pos[pending] + dist[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+            inside = np.all(np.abs(cand) < limit, axis=1)
+            out[pending[inside]] = cand[inside]
+            pending = pending[~inside]
+        return out
+
+    def reset(self, ids, seeds=None, targets=None):
+        """arc/task.py:108-125; seeds and first checkpoints given, or drawn
+        from self.rng (seeds first, then the checkpoints after the reset)"""
         ids = np.asarray(ids, dtype=np.int64)
         if not len(ids):
             return
+        if seeds is None:
+            seeds = self.rng.integers(0, 2**32, size=len(ids), dtype=np.uint32)
+        self.last_seeds = seeds
         self.difficulty[ids] = self.level[ids]
         self.sim.reset(ids, seeds, self.difficulty[ids])
         self.prev_action[ids] = 0.0
@@ -71,11 +96,12 @@ class OracleTask:
         self.steps[ids] = 0
         self.checkpoints[ids] = 0
         self._observe()
-        self._set_targets(ids, targets)
+        self._set_targets(ids, self._draw_targets(ids) if targets is None else targets)
 
-    def step(self, actions, new_targets):
+    def step(self, actions, new_targets=None):
         """arc/task.py:155-239; new_targets(ids) gives the checkpoints placed
-        for the worlds that reached theirs.  Returns reward, terminated,
+        for the worlds that reached theirs (None: drawn from self.rng).
+        Returns reward, terminated,
         truncated, stats, and the worlds that got a new checkpoint."""
         cfg, sim = self.cfg, self.sim
         a = np.clip(np.asarray(actions, dtype=np.float64), -1.0, 1.0)
@@ -118,7 +144,7 @@ class OracleTask:
         self.checkpoints += reached
         hit = np.flatnonzero(reached & ~fell)
         if len(hit):
-            self._set_targets(hit, new_targets(hit))
+            self._set_targets(hit, self._draw_targets(hit) if new_targets is None else new_targets(hit))
         terminated = fell
         truncated = (self.steps >= cfg.max_steps) & ~terminated
         stats = {"checkpoints": self.checkpoints.copy(), "difficulty": self.difficulty.copy(),

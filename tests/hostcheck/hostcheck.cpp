@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
+#include "pcg64.h"
 #include "reset_engine.h"
 #include "setup.h"
 
@@ -237,4 +238,26 @@ extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
         g2 = maxf_(g2, ground_part(M.terr, tl, R.footprint, R.cell, lane, 64));
     bad += memcmp(&g1, &g2, 4) != 0;
     return bad;
+}
+
+/* numpy's generator as the device task draws from it (pcg64.h): the seeded
+ * state and increment as hi/lo words (out4), and `count` 64-bit outputs at the
+ * offsets `ks` from the seeded state, each taken by jump-ahead. */
+extern "C" void hc_np_seed(const unsigned *words, int n_words, unsigned long long *out4)
+{
+    NpStream g;
+    np_seed(g, words, n_words);
+    out4[0] = (unsigned long long)(g.state >> 64);
+    out4[1] = (unsigned long long)g.state;
+    out4[2] = (unsigned long long)(g.inc >> 64);
+    out4[3] = (unsigned long long)g.inc;
+}
+
+extern "C" void hc_np_draws(const unsigned *words, int n_words, const unsigned long long *ks, int count,
+                            unsigned long long *out)
+{
+    NpStream g;
+    np_seed(g, words, n_words);
+    for (int i = 0; i < count; ++i)
+        out[i] = np_draw(g, g.state, ks[i]);
 }

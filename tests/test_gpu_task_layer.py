@@ -3,12 +3,17 @@ checkpoints, curriculum and the env's auto-reset order, against the numpy
 restatement of arc/task.py and arc/env.py (oracle/pytask.py) over the
 binary32 oracle sim.
 
-The reference draws reset seeds and checkpoint positions from numpy's PCG64;
-the device draws them from per-world xorshift128 streams.  The test reads the
-device's draws back and gives them to the oracle, which restates everything
-else.  The engine state is bit-exact between the two, the task arithmetic is
-binary64 on both sides (only the device's and glibc's atan2 / cos / sin /
-hypot may differ in the last bit), so: integers (terminations, causes,
+The reference draws reset seeds and checkpoint positions from numpy's PCG64.
+In the device's default mode they come from per-world xorshift128 streams: the
+test reads the device's draws back and gives them to the oracle, which
+restates everything else.  In numpy mode the device draws from
+np.random.default_rng(seed) in the reference's order and the oracle draws from
+numpy's own generator: reset seeds and the generator's state (position in the
+stream, 32-bit buffer) must then be equal after every step, checkpoints within
+1e-9 m (the device's and numpy's binary64 cos / sin).  The engine state is
+bit-exact between the two, the task arithmetic is binary64 on both sides (only
+the device's and glibc's atan2 / cos / sin / hypot may differ in the last
+bit), so: integers (terminations, causes,
 checkpoint counts, episode steps, reset lists) and curriculum levels exact;
 rewards and observations (float32) within 1e-5 (|b| + 1).  The tightened
 config makes checkpoints, falls, truncations and level changes frequent."""
@@ -29,21 +34,33 @@ def _close(a, b):
     return np.all(np.abs(a.astype(np.float64) - b.astype(np.float64)) <= TOL * (np.abs(b) + 1.0))
 
 
-@pytest.mark.parametrize("form", ["lane", "leg"])
-def test_device_task_matches_oracle(form):
+def _same_state(t, rng):
+    a, b = t.numpy_state(), rng.bit_generator.state
+    return (a["state"] == b["state"] and a["has_uint32"] == b["has_uint32"]
+            and (not b["has_uint32"] or a["uinteger"] == b["uinteger"]))
+
+
+@pytest.mark.parametrize("form,mode", [("lane", "world"), ("leg", "world"), ("leg", "numpy"), ("lane", "numpy")])
+def test_device_task_matches_oracle(form, mode):
     from arc import devtask
     n, steps = 96, 40
+    npm = mode == "numpy"
     cfg = devtask.TaskConfig(max_steps=17, checkpoint_radius=3.0, fov=120.0, beached_steps=2, flip_up_z=0.95,
                              level_up_checkpoints=1, level_down_steps=10)
-    t = devtask.DeviceTask(n, cfg, seed=3, form=form)
+    t = devtask.DeviceTask(n, cfg, seed=3, form=form, rng=mode)
     t.reset()
     t.sync()
     ids, seeds = t.last_resets()
     assert np.array_equal(ids, np.arange(n))
     tg = t.targets()
     o = OracleTask(P.Lib("o32").sim(n, cfg.dt, cfg.substeps, cfg.grid, cfg.cell), cfg, devtask.HEIGHT_OFFSETS,
-                   cfg.grid, cfg.cell)
-    o.reset(ids, seeds, tg[ids, :2])
+                   cfg.grid, cfg.cell, rng=np.random.default_rng(3) if npm else None)
+    if npm:
+        o.reset(ids)
+        assert np.array_equal(seeds, o.last_seeds) and _same_state(t, o.rng)
+        assert np.allclose(tg[:, :2], o.target, rtol=0.0, atol=1e-9)
+    else:
+        o.reset(ids, seeds, tg[ids, :2])
     assert np.array_equal(o.target_z, tg[:, 2])
     assert _close(t.obs.numpy(), o.observe())
     # first checkpoints: 3-6 m from the spawn point, inside the border
@@ -62,12 +79,19 @@ def test_device_task_matches_oracle(form):
         t.sync()
         tg = t.targets()
         lv_before = o.level.copy()
-        rd, term, trunc, stats, hit = o.step(a, lambda h: tg[h, :2])
+        rd, term, trunc, stats, hit = o.step(a, None if npm else (lambda h: tg[h, :2]))
         obs = o.observe()
         ended = np.flatnonzero(term | trunc)
         dids, dseeds = t.last_resets()
         assert np.array_equal(dids, ended), f"step {k}: reset lists"
-        o.reset(ended, dseeds, tg[ended, :2])
+        if npm:
+            o.reset(ended)
+            if len(ended):
+                assert np.array_equal(dseeds, o.last_seeds), f"step {k}: reset seeds"
+            assert _same_state(t, o.rng), f"step {k}: generator state"
+            assert np.allclose(tg[:, :2], o.target, rtol=0.0, atol=1e-9), f"step {k}: checkpoints"
+        else:
+            o.reset(ended, dseeds, tg[ended, :2])
         obs2 = o.observe()
         final = obs.copy()
         final[ended] = obs2[ended]
@@ -85,6 +109,34 @@ def test_device_task_matches_oracle(form):
         counts["fell"] += int(term.sum())
         counts["truncated"] += int(trunc.sum())
         counts["level_changes"] += int(np.sum(o.level != lv_before))
-    print(form, counts)
+    print(form, mode, counts)
     assert counts["reached"] > 0 and counts["fell"] > 0 and counts["truncated"] > 0 and counts["level_changes"] > 0
+    t.close()
+
+
+def test_numpy_stream_reseed_and_border_rounds():
+    """Re-seeding restarts the stream (ArcVecEnv.reset with a seed,
+    arc/env.py:93-95); worlds near the border need several placement rounds
+    and the 32-bit buffer carries across calls (odd reset counts)."""
+    from arc import devtask
+    n = 37
+    cfg = devtask.TaskConfig(border=8.8)  # |x|, |y| < 4 m: a candidate 3-6 m out is often outside
+    t = devtask.DeviceTask(n, cfg, seed=5, rng="numpy")
+    o = OracleTask(P.Lib("o32").sim(n, cfg.dt, cfg.substeps, cfg.grid, cfg.cell), cfg, devtask.HEIGHT_OFFSETS,
+                   cfg.grid, cfg.cell, rng=np.random.default_rng(5))
+    for rep in range(3):
+        t.reset()
+        t.sync()
+        o.reset(np.arange(n))
+        _, seeds = t.last_resets()
+        assert np.array_equal(seeds, o.last_seeds) and _same_state(t, o.rng), f"reset {rep}"
+        assert np.allclose(t.targets()[:, :2], o.target, rtol=0.0, atol=1e-9)
+    assert o.rng.bit_generator.state["has_uint32"] == 1  # 37 x 3 halves: one kept
+    assert np.any(o.target != 0.0)
+    t.seed_numpy(2**70 + 9)
+    o.rng = np.random.default_rng(2**70 + 9)
+    t.reset()
+    t.sync()
+    o.reset(np.arange(n))
+    assert np.array_equal(t.last_resets()[1], o.last_seeds) and _same_state(t, o.rng)
     t.close()
