@@ -57,8 +57,8 @@ struct ResetK {
     float stand, footprint;
     int grid;         /* generated grid size n (n x n columns) */
     float cell;
-    int chunk;        /* draws per lane of a generated reset (64 lanes) */
-    RngMatrix jump[JUMP_BITS]; /* chunk * 2^b draws ahead */
+    int chunk;        /* draws per lane of a generated reset (RESET_LANES lanes) */
+    const uint32_t *jtab; /* per-lane jump tables (reset_engine.h); device memory in the device copy */
 };
 
 /* Template state broadcast at creation (the robot as built, flat ground). */

@@ -101,15 +101,21 @@ int step_resident_blocks(int block, int tile, int leg)
 /* ------------------------------------------------------------------ */
 /* reset: terrain generation, joint noise, yaw, ground scan, FK          */
 
-__global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restrict__ m, const ResetK *__restrict__ rk,
-                                              const int *__restrict__ ids, const uint32_t *__restrict__ seeds,
-                                              const double *__restrict__ diff_d, const float *__restrict__ diff_f,
-                                              int n_ids, const int *__restrict__ count_dev)
+__global__ __launch_bounds__(RESET_LANES) void k_reset(DevWorlds d, const Model *__restrict__ m,
+                                                       const ResetK *__restrict__ rk, const int *__restrict__ ids,
+                                                       const uint32_t *__restrict__ seeds,
+                                                       const double *__restrict__ diff_d,
+                                                       const float *__restrict__ diff_f, int n_ids,
+                                                       const int *__restrict__ count_dev)
 {
     __shared__ float hbuf[GRID_LIMIT_CELLS];
     __shared__ int8_t qbuf[(64 + 2 * TPAD) * (64 + 2 * TPAD)];
     __shared__ float lat[LAT_LIMIT];
     __shared__ float draws[NJ + 1];
+    __shared__ float gpart[RESET_LANES / 64];
+    __shared__ float s_ang[NJ];
+    __shared__ Q4 s_rel[NJ];  /* qaxis(a1, angle) * qrel0 of each joint */
+    __shared__ float s_pose[NB * 7];
     const Model &M = *m;
     const ResetK &R = *rk;
     const int tid = threadIdx.x;
@@ -122,16 +128,24 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
     const size_t S = d.stride;
     const int pitch = d.tpitch;
 
-    /* the draws: one stream, cut into 64 contiguous chunks that the lanes
-     * reach by jumping ahead (without terrain: 13 draws, lane 0) */
+    /* the draws: one stream, cut into RESET_LANES contiguous chunks that the
+     * lanes reach by jumping ahead (without terrain: 13 draws, lane 0) */
     if (gen)
-        reset_draws_lane(seeds[k], gen, n, R.jump, R.chunk, tid, lat, hbuf, draws);
+        reset_draws_lane(seeds[k], gen, n, R.jtab, R.chunk, tid, lat, hbuf, draws);
     else if (tid == 0)
         reset_draws(seeds[k], gen, n, lat, hbuf, draws);
     __syncthreads();
     /* quantisation steps of the columns (csrc/arc_terrain.c:54-71) */
-    for (int c = tid; c < cells; c += 64)
+    for (int c = tid; c < cells; c += RESET_LANES)
         hbuf[c] = gen ? column_step(lat, n, dfc, hbuf[c], c) : 0.0f;
+    /* each joint's rotation from its angle (csrc/arc_sim.c:92-100), in
+     * parallel: the chain below then only multiplies */
+    if (tid < NJ) {
+        const JointK &J = M.joint[tid];
+        const float ang = clampf_(R.def[tid] + draws[tid], R.lo[tid], R.hi[tid]);
+        s_ang[tid] = ang;
+        s_rel[tid] = qmul(qaxis(J.a1, ang), J.qrel0);
+    }
     __syncthreads();
     /* base: the pre-shift height of the column at the origin */
     float base = 0.0f;
@@ -140,7 +154,7 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
         base = 0.03f * hbuf[ix + iy * n];
     }
     int8_t *gq = d.tq + (size_t)i * d.tcells;
-    for (int c = tid; c < d.tcells; c += 64) {
+    for (int c = tid; c < d.tcells; c += RESET_LANES) {
         int ix = c % pitch - TPAD, iy = c / pitch - TPAD;
         int8_t v = (ix >= 0 && iy >= 0 && ix < n && iy < n) ? (int8_t)(int)hbuf[ix + iy * n] : (int8_t)Q_OUT;
         qbuf[c] = v;
@@ -154,38 +168,67 @@ __global__ __launch_bounds__(64) void k_reset(DevWorlds d, const Model *__restri
     tl.outside = -100.0f;
 
     /* the footprint scan split over the lanes, then the maximum */
-    float ground = ground_part(M.terr, tl, R.footprint, R.cell, tid, 64);
+    float ground = ground_part(M.terr, tl, R.footprint, R.cell, tid, RESET_LANES);
     for (int off = 32; off; off >>= 1)
         ground = maxf_(ground, __shfl_xor(ground, off, 64));
+    if ((tid & 63) == 0)
+        gpart[tid >> 6] = ground;
+    __syncthreads();
+    /* forward kinematics (pose_bodies): the torso, then each leg's chain of
+     * three joints in its own lane */
+    if (tid < LEGS) {
+        float g = gpart[0];
+        for (int w = 1; w < RESET_LANES / 64; ++w)
+            g = maxf_(g, gpart[w]);
+        V3 px = v3(0.0f, 0.0f, g + R.stand + 0.03f);
+        Q4 pq = qunit(qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]));
+        if (tid == 0) {
+            s_pose[0] = px.x; s_pose[1] = px.y; s_pose[2] = px.z;
+            s_pose[3] = pq.w; s_pose[4] = pq.x; s_pose[5] = pq.y; s_pose[6] = pq.z;
+        }
+        for (int jj = 0; jj < 3; ++jj) {
+            const int j = 3 * tid + jj, c = joint_child(j);
+            const JointK &J = M.joint[j];
+            V3 anchor = add(px, qrot(pq, J.r1));
+            Q4 cq = qunit(qmul(pq, s_rel[j]));
+            V3 cx = sub(anchor, qrot(cq, J.r2));
+            float *o = s_pose + 7 * c;
+            o[0] = cx.x; o[1] = cx.y; o[2] = cx.z;
+            o[3] = cq.w; o[4] = cq.x; o[5] = cq.y; o[6] = cq.z;
+            px = cx;
+            pq = cq;
+        }
+    }
+    __syncthreads();
+    /* the world's state, one value per lane */
+    for (int e = tid; e < NB * 13; e += RESET_LANES) {
+        const int b = e / 13, f = e % 13;
+        if (f < 3)
+            d.x[(b * 3 + f) * S + i] = s_pose[7 * b + f];
+        else if (f < 7)
+            d.q[(b * 4 + f - 3) * S + i] = s_pose[7 * b + f];
+        else if (f < 10)
+            d.v[(b * 3 + f - 7) * S + i] = 0.0f;
+        else
+            d.w[(b * 3 + f - 10) * S + i] = 0.0f;
+    }
+    for (int e = tid; e < NJ * 4; e += RESET_LANES) {
+        const int j = e >> 2, f = e & 3;
+        if (f == 0)
+            d.target[j * S + i] = s_ang[j];
+        else
+            d.jout[((f - 1) * NJ + j) * S + i] = f == 1 ? s_ang[j] : 0.0f;
+    }
+    for (int e = tid; e < NS * 7; e += RESET_LANES) {
+        const int sp = e / 7, f = e % 7;
+        if (f == 0)
+            d.touch[sp * S + i] = 0;
+        else if (f < 4)
+            d.anc[(sp * 3 + f - 1) * S + i] = 0.0f;
+        else
+            d.ancn[(sp * 3 + f - 4) * S + i] = 0.0f;
+    }
     if (tid == 0) {
-        float ang[NJ];
-        for (int j = 0; j < NJ; ++j)
-            ang[j] = clampf_(R.def[j] + draws[j], R.lo[j], R.hi[j]);
-        V3 bx[NB];
-        Q4 bq[NB];
-        pose_bodies(M, v3(0.0f, 0.0f, ground + R.stand + 0.03f), qaxis(v3(0.0f, 0.0f, 1.0f), draws[NJ]), ang, bx, bq);
-        for (int b = 0; b < NB; ++b) {
-            d.x[(b * 3 + 0) * S + i] = bx[b].x; d.x[(b * 3 + 1) * S + i] = bx[b].y; d.x[(b * 3 + 2) * S + i] = bx[b].z;
-            d.q[(b * 4 + 0) * S + i] = bq[b].w; d.q[(b * 4 + 1) * S + i] = bq[b].x;
-            d.q[(b * 4 + 2) * S + i] = bq[b].y; d.q[(b * 4 + 3) * S + i] = bq[b].z;
-            for (int c = 0; c < 3; ++c) {
-                d.v[(b * 3 + c) * S + i] = 0.0f;
-                d.w[(b * 3 + c) * S + i] = 0.0f;
-            }
-        }
-        for (int j = 0; j < NJ; ++j) {
-            d.target[j * S + i] = ang[j];
-            d.jout[(0 * NJ + j) * S + i] = ang[j];
-            d.jout[(1 * NJ + j) * S + i] = 0.0f;
-            d.jout[(2 * NJ + j) * S + i] = 0.0f;
-        }
-        for (int s = 0; s < NS; ++s) {
-            d.touch[s * S + i] = 0;
-            for (int c = 0; c < 3; ++c) {
-                d.anc[(s * 3 + c) * S + i] = 0.0f;
-                d.ancn[(s * 3 + c) * S + i] = 0.0f;
-            }
-        }
         d.anchored[i] = 0;
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
@@ -205,7 +248,7 @@ hipError_t launch_reset(const DevWorlds &d, const Model *m_dev, const ResetK *r_
     int blocks = count_dev ? max_blocks : (n_ids < max_blocks || max_blocks <= 0 ? n_ids : max_blocks);
     if (blocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(64), 0, s, d, m_dev, r_dev, ids, seeds, difficulty_d, difficulty_f,
+    hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(RESET_LANES), 0, s, d, m_dev, r_dev, ids, seeds, difficulty_d, difficulty_f,
                        n_ids, count_dev);
     return hipGetLastError();
 }

@@ -146,17 +146,65 @@ XD void reset_draws(uint32_t seed, bool gen, int n, float *lat, float *jit, floa
     draws[NJ] = r.range(-3.14159265358979323846f, 3.14159265358979323846f);
 }
 
-/* Draws of a reset spread over `lanes` lanes: lane L makes the draws
- * [L * chunk, (L + 1) * chunk) of the stream, starting from the seeded state
- * moved ahead by the matrices jump[b] = (chunk * 2^b draws), b = 0..5.
- * Writes each draw where reset_draws puts it. */
-constexpr int JUMP_BITS = 6;
+/* Draws of a reset spread over the RESET_LANES lanes of its workgroup: lane
+ * L makes the draws [L * chunk, (L + 1) * chunk) of the stream, starting from
+ * the seeded state moved L * chunk draws ahead.  The generator is linear over
+ * GF(2)^128, so that move is one fixed 128 x 128 bit matrix per lane, applied
+ * from a table of 4-bit windows: entry [w][v] is the XOR of the matrix columns
+ * 4w..4w+3 selected by the bits of v, so a product is 32 lookups.  Writes
+ * each draw where reset_draws puts it. */
+constexpr int RESET_LANES = 256;
+constexpr int JTAB_WORDS = 32 * 16 * 4; /* one lane's table: 32 windows x 16 entries x 4 words */
+
 XD int reset_draw_count(bool gen, int n)
 {
     const int m1 = lattice1(n), m2 = lattice2(n);
     return (gen ? m1 * m1 + m2 * m2 + n * n : 0) + NJ + 1;
 }
-XD void reset_draws_lane(uint32_t seed, bool gen, int n, const RngMatrix *jump, int chunk, int lane, float *lat,
+XD int reset_chunk(int n) { return (reset_draw_count(true, n) + RESET_LANES - 1) / RESET_LANES; }
+
+XD void rng_apply_tab(const uint32_t *tab, uint32_t s[4])
+{
+    uint32_t r0 = 0u, r1 = 0u, r2 = 0u, r3 = 0u;
+#pragma unroll
+    for (int w = 0; w < 32; ++w) {
+        const uint32_t v = (s[w >> 3] >> (4 * (w & 7))) & 15u;
+        const uint32_t *e = tab + (w * 16 + (int)v) * 4;
+        r0 ^= e[0];
+        r1 ^= e[1];
+        r2 ^= e[2];
+        r3 ^= e[3];
+    }
+    s[0] = r0;
+    s[1] = r1;
+    s[2] = r2;
+    s[3] = r3;
+}
+
+/* the window tables of the matrices of L * chunk draws, L = 0..RESET_LANES-1
+ * (RESET_LANES * JTAB_WORDS words) */
+inline void rng_lane_tables(int chunk, uint32_t *out)
+{
+    RngMatrix step, m;
+    rng_power((uint64_t)chunk, step);
+    rng_power(0, m); /* identity */
+    for (int L = 0; L < RESET_LANES; ++L) {
+        uint32_t *t = out + (size_t)L * JTAB_WORDS;
+        for (int w = 0; w < 32; ++w)
+            for (int v = 0; v < 16; ++v) {
+                uint32_t e[4] = { 0u, 0u, 0u, 0u };
+                for (int b = 0; b < 4; ++b)
+                    if ((v >> b) & 1)
+                        for (int k = 0; k < 4; ++k)
+                            e[k] ^= m.col[4 * w + b][k];
+                for (int k = 0; k < 4; ++k)
+                    t[(w * 16 + v) * 4 + k] = e[k];
+            }
+        rng_compose(step, m, m);
+    }
+}
+
+XD void reset_draws_lane(uint32_t seed, bool gen, int n, const uint32_t *jtab, int chunk, int lane, float *lat,
                          float *jit, float *draws)
 {
     const int m1 = lattice1(n), m2 = lattice2(n);
@@ -167,9 +215,8 @@ XD void reset_draws_lane(uint32_t seed, bool gen, int n, const RngMatrix *jump, 
         return;
     Rng r;
     r.seed(seed);
-    for (int b = 0; b < JUMP_BITS; ++b)
-        if ((lane >> b) & 1)
-            rng_apply(jump[b], r.s);
+    if (lane > 0)
+        rng_apply_tab(jtab + (size_t)lane * JTAB_WORDS, r.s);
     const int e = a + chunk < total ? a + chunk : total;
     for (int q = a; q < e; ++q) {
         if (q < nl) {

@@ -134,11 +134,11 @@ bool sim_setup(SimSetup &S, double dt, int substeps, int grid, double cell)
     R.footprint = S.info.footprint;
     R.grid = S.grid;
     R.cell = (float)cell;
-    /* jump-ahead matrices of a generated reset's stream cut into 64 chunks */
-    R.chunk = (reset_draw_count(true, S.grid) + 63) / 64;
-    rng_power((uint64_t)R.chunk, R.jump[0]);
-    for (int b = 1; b < JUMP_BITS; ++b)
-        rng_compose(R.jump[b - 1], R.jump[b - 1], R.jump[b]);
+    /* jump tables of a generated reset's stream cut into RESET_LANES chunks */
+    R.chunk = reset_chunk(S.grid);
+    S.jtab.assign((size_t)RESET_LANES * JTAB_WORDS, 0u);
+    rng_lane_tables(R.chunk, S.jtab.data());
+    R.jtab = S.jtab.data();
 
     TemplateK &T = S.tk;
     for (int b = 0; b < NB; ++b) {

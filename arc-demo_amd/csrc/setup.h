@@ -6,6 +6,7 @@
  * harness build it with the same function.
  */
 #pragma once
+#include <vector>
 #include "device_state.h"
 #include "model.h"
 
@@ -17,6 +18,7 @@ struct SimSetup {
     RobotInfo info;
     Model model;
     ResetK rk;
+    std::vector<uint32_t> jtab; /* rk's jump tables on the host (rk.jtab points here) */
     TemplateK tk;
     int channel[MAX_SPHERES];
 };

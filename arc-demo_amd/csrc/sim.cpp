@@ -128,19 +128,29 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         return nullptr;
     }
     hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess
-        || (e = dev_alloc(s->dev, n_worlds, s->su.grid, M.cslots)) != hipSuccess
-        || (e = hipMalloc((void **)&s->d_model, sizeof(Model))) != hipSuccess
-        || (e = hipMalloc((void **)&s->d_reset, sizeof(ResetK))) != hipSuccess
-        || (e = hipMalloc((void **)&s->d_tmpl, sizeof(TemplateK))) != hipSuccess
-        || (e = hipMalloc((void **)&s->d_dump, 1024 * sizeof(float))) != hipSuccess
-        || (e = hipMalloc((void **)&s->d_wave, sizeof(WaveK))) != hipSuccess
-        || (e = hipMemcpyAsync(s->d_wave, &s->wave, sizeof(WaveK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
-        || (e = hipMemcpyAsync(s->d_model, &M, sizeof(Model), hipMemcpyHostToDevice, s->stream)) != hipSuccess
-        || (e = hipMemcpyAsync(s->d_reset, &s->su.rk, sizeof(ResetK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
-        || (e = hipMemcpyAsync(s->d_tmpl, &s->su.tk, sizeof(TemplateK), hipMemcpyHostToDevice, s->stream)) != hipSuccess
-        || (e = launch_init_template(s->dev, s->d_tmpl, s->su.tmpl.outside, s->stream)) != hipSuccess
-        || (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+    const size_t jtab_bytes = s->su.jtab.size() * sizeof(uint32_t);
+    ResetK rk_dev = s->su.rk; /* the device copy points at the device tables */
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) == hipSuccess
+        && (e = dev_alloc(s->dev, n_worlds, s->su.grid, M.cslots)) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_model, sizeof(Model))) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_reset, sizeof(ResetK))) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_jtab, jtab_bytes)) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_tmpl, sizeof(TemplateK))) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_dump, 1024 * sizeof(float))) == hipSuccess
+        && (e = hipMalloc((void **)&s->d_wave, sizeof(WaveK))) == hipSuccess) {
+        rk_dev.jtab = s->d_jtab;
+        if ((e = hipMemcpyAsync(s->d_jtab, s->su.jtab.data(), jtab_bytes, hipMemcpyHostToDevice, s->stream))
+                == hipSuccess
+            && (e = hipMemcpyAsync(s->d_wave, &s->wave, sizeof(WaveK), hipMemcpyHostToDevice, s->stream)) == hipSuccess
+            && (e = hipMemcpyAsync(s->d_model, &M, sizeof(Model), hipMemcpyHostToDevice, s->stream)) == hipSuccess
+            && (e = hipMemcpyAsync(s->d_reset, &rk_dev, sizeof(ResetK), hipMemcpyHostToDevice, s->stream)) == hipSuccess
+            && (e = hipMemcpyAsync(s->d_tmpl, &s->su.tk, sizeof(TemplateK), hipMemcpyHostToDevice, s->stream))
+                   == hipSuccess)
+            e = launch_init_template(s->dev, s->d_tmpl, s->su.tmpl.outside, s->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s->stream);
+    }
+    if (e != hipSuccess) {
         fprintf(stderr, "libarcsim: arc_sim_create: %s\n", hipGetErrorString(e));
         arc_sim_destroy(s);
         return nullptr;
@@ -188,6 +198,7 @@ void arc_sim_destroy(arc_sim *s)
     dev_free(s->dev);
     if (s->d_model) (void)hipFree(s->d_model);
     if (s->d_reset) (void)hipFree(s->d_reset);
+    if (s->d_jtab) (void)hipFree(s->d_jtab);
     if (s->d_tmpl) (void)hipFree(s->d_tmpl);
     if (s->d_dump) (void)hipFree(s->d_dump);
     if (s->d_wave) (void)hipFree(s->d_wave);

@@ -23,6 +23,7 @@ struct arc_sim {
     DevWorlds dev;
     Model *d_model;
     ResetK *d_reset;
+    uint32_t *d_jtab;      /* the reset's jump tables (ResetK::jtab of d_reset) */
     TemplateK *d_tmpl;
     hipStream_t stream;
     int device;

@@ -286,11 +286,20 @@ def rollout_pass(steps: int = 64, rollouts: int = 3, n: int = 8192) -> dict:
     2 x 256 tanh MLP in PyTorch reads the device task's observations and
     writes its actions, both exchanged zero-copy (DLPack out, device pointer
     in); rewards and dones gathered into a rollout buffer on the GPU.  Runs in
-    its own process (torch's HIP runtime must load before libarcsim)."""
+    its own process (torch's HIP runtime must load before libarcsim).  Timed
+    with per-world draw streams, then again with the draws from one numpy
+    generator in the reference's order."""
+    out = rollout_task(steps, rollouts, n, "world")
+    npm = rollout_task(steps, rollouts, n, "numpy")
+    out["numpy_stream"] = {k: npm[k] for k in ("value", "ms_per_step", "mean_reward", "done_fraction")}
+    return out
+
+
+def rollout_task(steps: int, rollouts: int, n: int, mode: str) -> dict:
     import torch  # noqa: F401  (first: libarcsim then binds torch's HIP runtime)
     from arc import devtask
     cfg = devtask.TaskConfig(curriculum=False)
-    t = devtask.DeviceTask(n, cfg, seed=0)
+    t = devtask.DeviceTask(n, cfg, seed=0, rng=mode)
     t.set_levels(unit24(rng(1, n)))
     t.reset()
     cfgl, _ = t.sim.autotune(3)
@@ -323,10 +332,12 @@ def rollout_pass(steps: int = 64, rollouts: int = 3, n: int = 8192) -> dict:
     for _ in range(rollouts):
         rollout()
     el = time.perf_counter() - t0
+    t.close()
     return {"value": n * steps * rollouts / el, "unit": "world-steps/s", "worlds": n, "rollout_steps": steps,
             "rollouts": rollouts, "ms_per_step": 1e3 * el / (steps * rollouts), "launch": cfgl,
             "obs_size": t.obs_size, "mean_reward": float(buf_rew.mean()), "done_fraction": float(buf_done.float().mean()),
-            "policy": "random-init MLP 195-256-256-12 tanh, fp32", "exchange": "DLPack (obs, rewards, dones), device pointer (actions)"}
+            "policy": "random-init MLP 195-256-256-12 tanh, fp32", "exchange": "DLPack (obs, rewards, dones), device pointer (actions)",
+            "draws": mode}
 
 
 def rollout_child() -> dict:

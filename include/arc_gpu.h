@@ -52,8 +52,10 @@ int arcx_step_device(arc_sim *s, const float *targets_dev);
 int arcx_sync(arc_sim *s);
 
 /* Benchmark helper: `steps` control steps back to back, step k reading
- * targets_dev + k * step_floats; kernel_ms = summed HIP-event time of the
- * step kernels, wall_ms = event time from before the first to after the last. */
+ * targets_dev + k * step_floats (the buffer must hold
+ * (steps - 1) * step_floats + N * 12 floats; step_floats = 0 repeats one set);
+ * kernel_ms = summed HIP-event time of the step kernels, wall_ms = event time
+ * from before the first to after the last. */
 int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long step_floats, double *kernel_ms,
                     double *wall_ms);
 

@@ -198,7 +198,7 @@ extern "C" void hc_joint_consts(const HC *h, int j, double *o)
     o[18] = k.qrel0.w; o[19] = k.qrel0.x; o[20] = k.qrel0.y; o[21] = k.qrel0.z;
 }
 
-/* The reset's draws made as 64 jump-ahead chunks (the device's way) against
+/* The reset's draws made as RESET_LANES jump-ahead chunks (the device's way) against
  * the one sequential stream, and the split footprint scan against the plain
  * one.  Returns the number of differing values (0 expected). */
 extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
@@ -210,8 +210,8 @@ extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
     float d1[NJ + 1], d2[NJ + 1];
     reset_draws(seed, gen != 0, n, lat.data(), jit.data(), d1);
     if (gen)
-        for (int lane = 0; lane < 64; ++lane)
-            reset_draws_lane(seed, true, n, R.jump, R.chunk, lane, lat2.data(), jit2.data(), d2);
+        for (int lane = 0; lane < RESET_LANES; ++lane)
+            reset_draws_lane(seed, true, n, h->su.jtab.data(), R.chunk, lane, lat2.data(), jit2.data(), d2);
     else
         reset_draws(seed, false, n, lat2.data(), jit2.data(), d2);
     int bad = 0;
@@ -234,8 +234,8 @@ extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
     tl.base = 0.09f;
     tl.outside = -100.0f;
     float g1 = ground_under(M.terr, tl, R.footprint, R.cell), g2 = -1e30f;
-    for (int lane = 0; lane < 64; ++lane)
-        g2 = maxf_(g2, ground_part(M.terr, tl, R.footprint, R.cell, lane, 64));
+    for (int lane = 0; lane < RESET_LANES; ++lane)
+        g2 = maxf_(g2, ground_part(M.terr, tl, R.footprint, R.cell, lane, RESET_LANES));
     bad += memcmp(&g1, &g2, 4) != 0;
     return bad;
 }

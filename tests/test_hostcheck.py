@@ -81,9 +81,10 @@ def test_unstable_world_frozen():
 
 @pytest.mark.parametrize("grid,cell", [(64, 0.4), (48, 0.4), (20, 0.25), (8, 0.4)])
 def test_reset_split_matches_sequential(grid, cell):
-    """The device's reset: draws made by 64 lanes jumping ahead in the
-    xorshift128 stream (GF(2) matrix powers), and the footprint scan split
-    over the lanes, equal the sequential stream and scan bit for bit."""
+    """The device's reset: draws made by 256 lanes jumping ahead in the
+    xorshift128 stream (each lane's GF(2) matrix power applied from 4-bit
+    window tables), and the footprint scan split over the lanes, equal the
+    sequential stream and scan bit for bit."""
     import ctypes
     h = HostCheck(1, grid=grid, cell=cell)
     h.L.hc_reset_split_check.restype = ctypes.c_int
