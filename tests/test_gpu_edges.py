@@ -1,0 +1,139 @@
+"""Edge cases of the step path on the GPU, through the C ABI.
+
+- world counts that fill no whole wavefront or quad group (1, 3, 17, 100):
+  every form (lane, four lanes per world, two lanes per world, one world per
+  wavefront) against the FP32 oracle after one step, and bit-identical to each
+  other;
+- reset inputs: an empty list (nothing changes), repeated ids (the last entry
+  wins, as resetting in list order does), ids out of range (skipped, the
+  others reset, the error reported);
+- the largest batch: 256 requests of uneven sizes, some of one world, step
+  exactly what one step of all worlds does;
+- the full size, 65,536 worlds: the forms agree bit for bit after a few steps
+  and a repeated run gives the same bits (size-independent properties, at the
+  shape the benchmark runs).
+"""
+
+import hashlib
+
+import numpy as np
+import pytest
+
+import pyoracle as P
+from helpers import action_stream, same
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def native():
+    from arc import native
+    return native
+
+
+def _states(sim, n):
+    return np.stack([sim.state(i) for i in range(n)])
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 100])
+def test_ragged_world_counts_all_forms(native, n):
+    diff = P.difficulties(2, n)
+    ids = np.arange(n, dtype=np.int32)
+    seeds = (np.arange(n) + 7).astype(np.uint32)
+    t = action_stream(n, 1)[0]
+    o = P.Lib("o32").sim(n)
+    o.reset(ids, seeds, diff)
+    o.step(t)
+    b = _states(o, n)
+    ref = None
+    for form in ("lane", "leg", "pair", "wave"):
+        g = native.Sim(n)
+        g.set_form(form)
+        g.reset(ids, seeds, diff)
+        g.step(t)
+        a = _states(g, n)
+        assert np.all(np.abs(a - b) <= REL * (np.abs(b) + 1.0)), f"{form}, {n} worlds"
+        if ref is None:
+            ref = a
+        else:
+            assert same(a, ref).all(), f"{form} differs from the lane form at {n} worlds"
+        g.close()
+
+
+def test_reset_edge_inputs(native):
+    n = 8
+    g = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), np.full(n, 0.6))
+    before = _states(g, n)
+    terr = [g.terrain(i) for i in range(n)]
+    # empty list: nothing changes
+    g.reset(np.zeros(0, np.int32), np.zeros(0, np.uint32), np.zeros(0))
+    assert same(_states(g, n), before).all()
+    # repeated ids: the last entry per world wins
+    g.reset(np.array([2, 5, 2], np.int32), np.array([100, 101, 102], np.uint32), np.array([0.3, 0.9, 1.0]))
+    h = native.Sim(n)
+    h.reset(ids, ids.astype(np.uint32), np.full(n, 0.6))
+    h.reset(np.array([2, 5], np.int32), np.array([102, 101], np.uint32), np.array([1.0, 0.9]))
+    assert same(_states(g, n), _states(h, n)).all()
+    assert np.array_equal(g.terrain(2), h.terrain(2)) and not np.array_equal(g.terrain(2), terr[2])
+    # out-of-range ids are skipped and reported; the valid ones are reset
+    with pytest.raises(native.SimError):
+        g.reset(np.array([-1, 3, n], np.int32), np.array([1, 55, 2], np.uint32), np.array([0.5, 0.5, 0.5]))
+    h.reset(np.array([3], np.int32), np.array([55], np.uint32), np.array([0.5]))
+    assert same(_states(g, n), _states(h, n)).all()
+    g.close()
+    h.close()
+
+
+def test_largest_batch_equals_full_step(native):
+    rng = np.random.default_rng(5)
+    counts = rng.integers(1, 40, size=256)
+    counts[::7] = 1  # single-world requests among them
+    n = int(counts.sum())
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    order = rng.permutation(256)  # requests in any order
+    ids = np.arange(n, dtype=np.int32)
+    diff = P.difficulties(3, n)
+    t = action_stream(n, 1)[0].astype(np.float32)
+    for form in ("lane", "leg"):
+        g = native.Sim(n)
+        h = native.Sim(n)
+        for s in (g, h):
+            s.set_form(form)
+            s.reset(ids, ids.astype(np.uint32), diff)
+        buf = native.DeviceBuffer.from_array(np.ascontiguousarray(t))
+        g.step_batch(starts[order], counts[order], buf.ptr)
+        g.sync()
+        h.step(t.astype(np.float64))
+        assert same(_states(g, n), _states(h, n)).all(), form
+        g.close()
+        h.close()
+
+
+def _run_full(native, form, steps, n=65536):
+    s = native.Sim(n)
+    s.set_form(form)
+    u = native.rng_stream(1, n)
+    s.reset(np.arange(n), np.arange(n), (u >> 8) * (1.0 / 16777216.0))
+    a = (native.rng_stream(0, n * 12 * steps) >> 8) * (1.0 / 8388608.0) - 1.0
+    host = (np.tile(s.default_angles, n * steps) + 0.6 * a).astype(np.float32)
+    buf = native.DeviceBuffer.from_array(host)
+    # step k reads floats [k n 12, (k + 1) n 12): all inside the buffer
+    s.time_steps(buf.ptr, steps, n * 12)
+    root, joints, contacts, flags = s.observe()
+    h = hashlib.sha256()
+    for x in (root, joints, contacts, flags):
+        h.update(np.ascontiguousarray(x).tobytes())
+    s.close()
+    return h.hexdigest(), int(flags.sum()), float(np.abs(root[:, 7:10]).mean())
+
+
+def test_full_size_forms_agree_and_repeat(native):
+    ref = _run_full(native, "leg", 4)
+    assert ref[1] == 0 and ref[2] > 0.0  # stable, moving
+    assert _run_full(native, "leg", 4) == ref
+    assert _run_full(native, "lane", 4) == ref
+    assert _run_full(native, "pair", 4) == ref
