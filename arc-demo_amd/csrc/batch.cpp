@@ -105,9 +105,10 @@ static BatchState *batch_of(arc_sim *s)
 }
 
 /* the group forms: 1 = four lanes per world (one leg each), 2 = two lanes
- * per world (two legs each); 0 = one world per lane */
-static int is_leg(const arc_sim *s) { return s->form == 2 ? 1 : s->form == 3 ? 2 : 0; }
-static int lanes_per_world(const arc_sim *s) { return s->form == 2 ? 4 : s->form == 3 ? 2 : 1; }
+ * per world (two legs each), 4 = four lanes per world with the root joints
+ * of a block's worlds solved by one wavefront; 0 = one world per lane */
+static int is_leg(const arc_sim *s) { return s->form == 2 ? 1 : s->form == 3 ? 2 : s->form == 4 ? 4 : 0; }
+static int lanes_per_world(const arc_sim *s) { return s->form == 2 || s->form == 4 ? 4 : s->form == 3 ? 2 : 1; }
 /* LDS layout actually launched: the group forms always hold the terrain tile */
 static int tile_of(const arc_sim *s) { return is_leg(s) ? 16 : s->tile; }
 
@@ -122,7 +123,13 @@ hipError_t ensure_leg(arc_sim *s)
     if (M.cslots > MAXC)
         return hipErrorNotSupported;
     const size_t slots = (size_t)std::max(leg_world_slots(M, 1), leg_world_slots(M, 2));
-    return hipMalloc((void **)&s->dev.lcbuf, slots * 6 * s->dev.stride * sizeof(float));
+    /* and the shared-roots form's dummy lists and anchors: one lane's slots
+     * and the 21 spheres' anchors for each of a block's 256 lanes */
+    const size_t dummy = ((size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]) * 6 + 2 * NS * 3) * 256;
+    hipError_t e = hipMalloc((void **)&s->dev.lcbuf, (slots * 6 * s->dev.stride + dummy) * sizeof(float));
+    if (e == hipSuccess)
+        s->dev.lcdummy = s->dev.lcbuf + slots * 6 * s->dev.stride;
+    return e;
 }
 
 /* grid of a launch over `total` worlds (graph: the worst case N) */
@@ -235,6 +242,8 @@ int arcx_set_layout(arc_sim *s, int tile)
 
 int arcx_set_persistent(arc_sim *s, int on)
 {
+    if (on && s->form == 4)
+        return -1; /* the shared-roots form moves whole blocks, not wavefronts */
     s->persistent = on ? 1 : 0;
     return 0;
 }
@@ -370,12 +379,15 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
         bool ok = hipMemcpyAsync(dt, ht.data(), nt * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
                   && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
                   && hipStreamSynchronize(s->stream) == hipSuccess;
-        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[3] = { 0, 2, 3 };
+        const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[4] = { 0, 2, 3, 4 };
         const bool leg_ok = ensure_leg(s) == hipSuccess; /* else lane form only */
-        for (int fi = 0; fi < 3 && ok; ++fi)
+        for (int fi = 0; fi < 4 && ok; ++fi)
         for (int bi = 0; bi < 3 && ok && (forms[fi] == 0 || leg_ok); ++bi)
             for (int ti = 0; ti < (forms[fi] == 0 ? 2 : 1) && ok; ++ti)
                 for (int pi = 0; pi < 2 && ok; ++pi) {
+                    /* the shared-roots form: blocks of 128 or 256, no work counter */
+                    if (forms[fi] == 4 && (blocks[bi] == 64 || pi))
+                        continue;
                     s->form = forms[fi];
                     s->block = blocks[bi];
                     s->tile = tiles[ti];

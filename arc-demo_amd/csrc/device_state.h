@@ -48,6 +48,8 @@ struct DevWorlds {
     float *difficulty; /* [stride] terrain difficulty of the last reset */
     float *cbuf;      /* [cslots*6][stride] per-substep contact list */
     float *lcbuf;     /* leg form's contact lists, [slot*6+field][4*stride] (allocated on first use) */
+    float *lcdummy;   /* shared-roots form: contact lists [slot*6+field][256] and anchors [2][NS*3][256]
+                         of lanes that step no world (block tails, frozen worlds) */
     int *err;         /* [stride] contact-list overflow bits (sticky) */
 };
 

@@ -42,6 +42,7 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
     d.base = p;
     d.bytes = bytes;
     d.lcbuf = nullptr;
+    d.lcdummy = nullptr;
     float *f = (float *)p;
     d.x = f; f += NB * 3 * S;
     d.q = f; f += NB * 4 * S;

@@ -153,13 +153,157 @@ template <int P, int T> __device__ __forceinline__ void bcast_vel(Body &B)
     gbcast3<P, T>(B.w);
 }
 
+/* The four root (hip) joints, which all move the torso: solved in turn, each
+ * by the lane holding its leg, the torso broadcast to the group after each
+ * owner's last root (positions, then velocities). */
+template <int K> struct RootsLocal {
+    __device__ __forceinline__ void positions(LegWorld<K> &W, int q, const Model &M)
+    {
+        constexpr int P = LEGS / K;
+        static_for<0, LEGS>([&](auto t) {
+            constexpr int owner = t / K, l = t % K;
+            if (q == owner) {
+                ROBOT_JOINT(3 * t, M);
+                joint_positions<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, W.target[3 * l],
+                                            M.hh, W.tacc[3 * l]);
+            }
+            if constexpr (l == K - 1)
+                bcast_pose<P, owner>(W.T);
+        });
+    }
+    __device__ __forceinline__ void velocities(LegWorld<K> &W, int q, const Model &M)
+    {
+        constexpr int P = LEGS / K;
+        static_for<0, LEGS>([&](auto t) {
+            constexpr int owner = t / K, l = t % K;
+            if (q == owner) {
+                ROBOT_JOINT(3 * t, M);
+                joint_velocities<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, M.h,
+                                             W.tacc[3 * l]);
+            }
+            if constexpr (l == K - 1)
+                bcast_vel<P, owner>(W.T);
+        });
+    }
+};
+
+/* The root joints of a workgroup's WPB worlds (one leg per lane, K = 1)
+ * solved by WPB lanes, one world each, through LDS: every lane publishes its
+ * coxa (and lane 0 of the group the torso), lane r of the block solves the
+ * four roots of world r in joint order, and every lane takes the torso and its
+ * coxa back.  The roots then use all lanes of one wavefront instead of one lane
+ * in four of every wavefront.  xs: [field][WPB] floats, RS_FIELDS fields. */
+constexpr int RS_FIELDS = 88;
+template <int WPB> struct RootsShared {
+    float *xs;
+    int g;        /* the lane's world slot in the block */
+    int turn = 0; /* the wavefront whose first WPB lanes solve the roots: turns by substep, spreading the
+                     chains over the SIMDs */
+    __device__ __forceinline__ bool solver() const { return (int)(threadIdx.x / 64) == turn && (int)(threadIdx.x % 64) < WPB; }
+    static constexpr int F_T = 0, F_C = 16, F_TG = 80, F_TA = 84; /* torso, coxas, targets, torque sums */
+    __device__ __forceinline__ float &at(int f, int w) const { return xs[f * WPB + w]; }
+    __device__ __forceinline__ void put_pose(int f, int w, const Body &b) const
+    {
+        at(f + 0, w) = b.x.x; at(f + 1, w) = b.x.y; at(f + 2, w) = b.x.z;
+        at(f + 3, w) = b.q.w; at(f + 4, w) = b.q.x; at(f + 5, w) = b.q.y; at(f + 6, w) = b.q.z;
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            at(f + 7 + e, w) = b.R.e[e / 3][e % 3];
+    }
+    __device__ __forceinline__ void get_pose(int f, int w, Body &b) const
+    {
+        b.x = v3(at(f + 0, w), at(f + 1, w), at(f + 2, w));
+        b.q = q4(at(f + 3, w), at(f + 4, w), at(f + 5, w), at(f + 6, w));
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            b.R.e[e / 3][e % 3] = at(f + 7 + e, w);
+    }
+    __device__ __forceinline__ void put_vel(int f, int w, const Body &b, bool with_r) const
+    {
+        at(f + 0, w) = b.w.x; at(f + 1, w) = b.w.y; at(f + 2, w) = b.w.z;
+        if (with_r)
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                at(f + 7 + e, w) = b.R.e[e / 3][e % 3];
+    }
+    __device__ __forceinline__ void get_vel(int f, int w, Body &b, bool with_r) const
+    {
+        b.w = v3(at(f + 0, w), at(f + 1, w), at(f + 2, w));
+        if (with_r)
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                b.R.e[e / 3][e % 3] = at(f + 7 + e, w);
+    }
+    __device__ __forceinline__ void positions(LegWorld<1> &W, int q, const Model &M)
+    {
+        __syncthreads(); /* the previous exchange has been read */
+        put_pose(F_C + 16 * q, g, W.b[0][0]);
+        if (q == 0)
+            put_pose(F_T, g, W.T);
+        at(F_TG + q, g) = W.target[0];
+        at(F_TA + q, g) = W.tacc[0];
+        __syncthreads();
+        if (solver()) {
+            /* one coxa in registers at a time beside the torso */
+            const int r = threadIdx.x % 64;
+            Body T;
+            get_pose(F_T, r, T);
+            static_for<0, LEGS>([&](auto t) {
+                Body C;
+                get_pose(F_C + 16 * t, r, C);
+                float ta = at(F_TA + t, r);
+                ROBOT_JOINT(3 * t, M);
+                joint_positions<KISO, KISO>(&T, &robot::body[0], C, robot::body[1], J, at(F_TG + t, r), M.hh, ta);
+                put_pose(F_C + 16 * t, r, C);
+                at(F_TA + t, r) = ta;
+            });
+            put_pose(F_T, r, T);
+        }
+        __syncthreads();
+        get_pose(F_T, g, W.T);
+        get_pose(F_C + 16 * q, g, W.b[0][0]);
+        W.tacc[0] = at(F_TA + q, g);
+    }
+    __device__ __forceinline__ void velocities(LegWorld<1> &W, int q, const Model &M)
+    {
+        __syncthreads();
+        put_vel(F_C + 16 * q, g, W.b[0][0], false);
+        if (q == 0)
+            put_vel(F_T, g, W.T, true);
+        at(F_TA + q, g) = W.tacc[0];
+        __syncthreads();
+        if (solver()) {
+            const int r = threadIdx.x % 64;
+            Body T;
+            get_vel(F_T, r, T, true);
+            static_for<0, LEGS>([&](auto t) {
+                Body C;
+                get_vel(F_C + 16 * t, r, C, false);
+                float ta = at(F_TA + t, r);
+                ROBOT_JOINT(3 * t, M);
+                joint_velocities<KISO, KISO>(&T, &robot::body[0], C, robot::body[1], J, M.h, ta);
+                at(F_C + 16 * t + 0, r) = C.w.x;
+                at(F_C + 16 * t + 1, r) = C.w.y;
+                at(F_C + 16 * t + 2, r) = C.w.z;
+                at(F_TA + t, r) = ta;
+            });
+            at(F_T + 0, r) = T.w.x; at(F_T + 1, r) = T.w.y; at(F_T + 2, r) = T.w.z;
+        }
+        __syncthreads();
+        W.T.w = v3(at(F_T + 0, g), at(F_T + 1, g), at(F_T + 2, g));
+        W.b[0][0].w = v3(at(F_C + 16 * q + 0, g), at(F_C + 16 * q + 1, g), at(F_C + 16 * q + 2, g));
+        W.tacc[0] = at(F_TA + q, g);
+        turn = (turn + 1) % (4 * WPB / 64); /* the block's wavefronts */
+    }
+};
+
 /* One control step of the group's world by lane q (legs q K .. q K + K - 1).
  * Anchors: global sphere indices; Poses: local bodies 0 torso, 1 + 3 l + m;
  * the contact store addresses this lane's own list.  Returns this lane's
  * contact-overflow bits (bit per global contact body, as the lane form). */
-template <int K, class Anchors, class Poses, class Tile>
+template <int K, class Anchors, class Poses, class Tile, class Roots>
 __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Poses &P0, const Model &M,
-                                        const TerrainRef &tr0, const ContactStore &cs0, const Tile &T0)
+                                        const TerrainRef &tr0, const ContactStore &cs0, const Tile &T0, Roots &RJ)
 {
     constexpr int P = LEGS / K;
     int overflow = 0;
@@ -191,16 +335,7 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
         /* joint positions: the four roots in turn (leg t's in lane t / K),
          * the torso broadcast when the next root is another lane's; then
          * each leg's mid and distal joints */
-        static_for<0, LEGS>([&](auto t) {
-            constexpr int owner = t / K, l = t % K;
-            if (q == owner) {
-                ROBOT_JOINT(3 * t, M);
-                joint_positions<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, W.target[3 * l],
-                                            M.hh, W.tacc[3 * l]);
-            }
-            if constexpr (l == K - 1)
-                bcast_pose<P, owner>(W.T);
-        });
+        RJ.positions(W, q, M);
         static_for<0, K>([&](auto l) {
             {
                 const JointK J = leg_joint<1>(q * K + l, M);
@@ -255,16 +390,7 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
             velocities(W.b[l][2], P0.load(3 + 3 * l), M.inv_h, M.two_h);
         });
 
-        static_for<0, LEGS>([&](auto t) {
-            constexpr int owner = t / K, l = t % K;
-            if (q == owner) {
-                ROBOT_JOINT(3 * t, M);
-                joint_velocities<KISO, KISO>(&W.T, &robot::body[0], W.b[l][0], robot::body[1], J, M.h,
-                                             W.tacc[3 * l]);
-            }
-            if constexpr (l == K - 1)
-                bcast_vel<P, owner>(W.T);
-        });
+        RJ.velocities(W, q, M);
         static_for<0, K>([&](auto l) {
             {
                 const JointK J = leg_joint<1>(q * K + l, M);

@@ -570,17 +570,22 @@ int arcx_sync(arc_sim *s)
 
 int arcx_set_form(arc_sim *s, int form)
 {
-    if (form < 0 || form > 3)
+    if (form < 0 || form > 4)
         return -1;
     if (form >= 2)
         HIPCHK_RET(s, ensure_leg(s), "arcx_set_form: group-form buffers", -1);
     s->form = form;
+    if (form == 4) { /* blocks of 128 or 256 threads, no work counter */
+        if (s->block == 64)
+            s->block = 256;
+        s->persistent = 0;
+    }
     return 0;
 }
 
 int arcx_set_launch(arc_sim *s, int block, int grid_blocks)
 {
-    if ((block != 64 && block != 128 && block != 256) || grid_blocks < 0)
+    if ((block != 64 && block != 128 && block != 256) || grid_blocks < 0 || (s->form == 4 && block == 64))
         return -1;
     s->block = block;
     s->grid_blocks = grid_blocks;

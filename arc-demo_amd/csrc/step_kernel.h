@@ -346,10 +346,12 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
         const LdsTile<WPB, TILE> tile = fill_tile<WPB, TILE>(s_tile + threadIdx.x / P, d, M.terr, (size_t)i,
                                                              d.x[0 * S + i], d.x[1 * S + i]);
         load_leg<K>(d, i, q, W);
-        ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile);
+        RootsLocal<K> roots;
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile, roots);
     } else {
         load_leg<K>(d, i, q, W);
-        ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile());
+        RootsLocal<K> roots;
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile(), roots);
     }
     store_leg<K>(d, opaque_index((size_t)i), q, W, ov);
 }
@@ -391,6 +393,80 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
     }
 }
 
+/* Four lanes per world with the root joints of the block's BLOCK / 4 worlds
+ * solved by the first lanes of one wavefront (RootsShared).  Every lane of
+ * the block takes part in the exchanges, so lanes without a world of their own
+ * (the last block's tail, worlds frozen by instability) step a copy of a real
+ * world's state with their anchors and contact lists in the dummy area and
+ * store nothing. */
+template <int BLOCK>
+__device__ __forceinline__ void step_world_shared(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+                                                  LdsPoses<BLOCK> &P0, int8_t *s_tile, RootsShared<BLOCK / 4> &roots,
+                                                  int i, int q, bool valid)
+{
+    constexpr int WPB = BLOCK / 4;
+    const size_t S = d.stride;
+    LegWorld<1> W;
+    for (int m = 0; m < 3; ++m) {
+        const int j = 3 * q + m;
+        W.target[m] = clampf_(targets[(size_t)i * NJ + j], M.joint[j].lo, M.joint[j].hi);
+        if (valid)
+            d.target[j * S + i] = W.target[m];
+    }
+    const bool live = valid && !d.unstable[i];
+    /* the dummy area holds 256 lanes' lists and anchors */
+    const float *dummy_anc = d.lcdummy + (size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]) * 6 * 256;
+    GlobalAnchors A = live ? GlobalAnchors{ d.anc, d.ancn, S, (size_t)i }
+                           : GlobalAnchors{ (float *)dummy_anc, (float *)dummy_anc + NS * 3 * 256, (size_t)256,
+                                            (size_t)threadIdx.x };
+    TerrainRef tr = terrain_ref(d, i);
+    tr.base = d.tbase[i];
+    tr.outside = d.outside[i];
+    ContactStore cs;
+    cs.base = live ? d.lcbuf : d.lcdummy;
+    cs.stride = live ? 4 * S : (size_t)256;
+    cs.world = live ? (size_t)i * 4 + q : (size_t)threadIdx.x;
+    const LdsTile<WPB, 16> tile = fill_tile<WPB, 16>(s_tile + threadIdx.x / 4, d, M.terr, (size_t)i, d.x[0 * S + i],
+                                                     d.x[1 * S + i]);
+    load_leg<1>(d, i, q, W);
+    const int ov = leg_step<1>(W, q, A, P0, M, tr, cs, tile, roots);
+    if (live)
+        store_leg<1>(d, opaque_index((size_t)i), q, W, ov);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_step_shared(
+    DevWorlds d, const Model *__restrict__ m, const float *__restrict__ targets, const StepBatch *__restrict__ batch,
+    int total)
+{
+    static_assert(BLOCK == 128 || BLOCK == 256, "blocks of two or four wavefronts");
+    constexpr int WPB = BLOCK / 4;
+    __shared__ float s_pose[4 * 7 * BLOCK];
+    __shared__ int8_t s_tile[16 * 16 * WPB];
+    __shared__ float s_roots[RS_FIELDS * WPB];
+    const Model &M = *m;
+    LdsPoses<BLOCK> P0;
+    P0.p = s_pose;
+    P0.tid = threadIdx.x;
+    RootsShared<WPB> roots;
+    roots.xs = s_roots;
+    roots.g = threadIdx.x >> 2;
+    const int n_ranges = batch ? batch->n_ranges : 0;
+    if (batch)
+        total = batch->total;
+    if (total <= 0)
+        return;
+    const int items = 4 * total;
+    /* whole blocks move together: every lane makes the same exchanges */
+    for (int base = (int)blockIdx.x * BLOCK; base < items; base += (int)gridDim.x * BLOCK) {
+        const int t = base + (int)threadIdx.x;
+        const bool valid = t < items;
+        const int w = (valid ? t : items - 1) >> 2;
+        step_world_shared<BLOCK>(d, M, targets, P0, s_tile, roots, batch ? batch_world(batch, n_ranges, w) : w,
+                                 (int)threadIdx.x & 3, valid);
+    }
+}
+
 /* workgroups of the BLOCK-thread kernel resident on the whole device at once */
 template <int BLOCK>
 int step_resident_blocks_of(int tile, int leg)
@@ -400,7 +476,13 @@ int step_resident_blocks_of(int tile, int leg)
         || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
     hipError_t e;
-    if (leg == 1)
+    if (leg == 4) {
+        if constexpr (BLOCK == 128 || BLOCK == 256)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_shared<BLOCK>, BLOCK, 0);
+        else
+            e = hipErrorInvalidValue;
+    }
+    else if (leg == 1)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16, 1>, BLOCK, 0);
     else if (leg == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_step_leg<BLOCK, 16, 2>, BLOCK, 0);
@@ -417,7 +499,7 @@ template <int BLOCK>
 hipError_t launch_step_block(const DevWorlds &d, const Model *m_dev, const float *targets, const StepBatch *batch,
                              int total, int tile, int grid_blocks, int *work, int leg, hipStream_t s)
 {
-    const int per_world = leg == 1 ? 4 : leg == 2 ? 2 : 1;
+    const int per_world = leg == 1 || leg == 4 ? 4 : leg == 2 ? 2 : 1;
     if (grid_blocks <= 0)
         grid_blocks = (per_world * total + BLOCK - 1) / BLOCK;
     if (grid_blocks <= 0)
@@ -427,6 +509,18 @@ hipError_t launch_step_block(const DevWorlds &d, const Model *m_dev, const float
          * picked them without it) */
         if (!d.lcbuf || tile != 16)
             return hipErrorInvalidValue;
+        if (leg == 4) {
+            /* whole blocks only; no work counter (the exchanges need every lane) */
+            if constexpr (BLOCK == 128 || BLOCK == 256) {
+                if (work || !d.lcdummy)
+                    return hipErrorInvalidValue;
+                hipLaunchKernelGGL(k_step_shared<BLOCK>, dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets,
+                                   batch, total);
+                return hipGetLastError();
+            } else {
+                return hipErrorInvalidValue;
+            }
+        }
         if (leg == 1)
             hipLaunchKernelGGL((k_step_leg<BLOCK, 16, 1>), dim3(grid_blocks), dim3(BLOCK), 0, s, d, m_dev, targets,
                                batch, total, work);

@@ -78,7 +78,11 @@ int arcx_task_episodes(const arc_sim *s, unsigned int *out);
  * of a control step in one launch, world state in registers), 1 = the plain
  * form, one world per wavefront with the world staged in LDS, 2 = four lanes
  * per world (one leg each, torso replicated; shortest per-world latency),
- * 3 = two lanes per world (two legs each). */
+ * 3 = two lanes per world (two legs each), 4 = four lanes per world with the
+ * four root joints of each block's worlds solved by the lanes of one wavefront
+ * through LDS (blocks of 128 or 256 threads, no persistent distribution: form
+ * 4 sets them, and arcx_set_launch / arcx_set_persistent refuse others while
+ * it is on). */
 int arcx_set_form(arc_sim *s, int form);
 
 /* Launch shape of the step kernel: threads per block (a multiple of 64) and

@@ -158,12 +158,13 @@ def test_batch_rejects_bad_requests(native):
         g.step_batch(list(range(257)), [0] * 257, dev.ptr)  # too many requests
 
 
-@pytest.mark.parametrize("form,tile", [("leg", 16), ("leg", 0), ("pair", 16)])
+@pytest.mark.parametrize("form,tile", [("leg", 16), ("leg", 0), ("pair", 16), ("shared", 16)])
 def test_leg_form_matches_lane_form(native, form, tile):
     """Four lanes per world (one leg each) and two lanes per world (two legs
-    each), torso replicated, hip joints chained through DPP broadcasts,
-    against one world per lane: bit-identical states, flags, contact counts
-    and joint outputs over several steps."""
+    each), torso replicated, hip joints chained through DPP broadcasts or
+    solved by one wavefront per block through LDS (shared; 200 worlds leave a
+    partial last block), against one world per lane: bit-identical states,
+    flags, contact counts and joint outputs over several steps."""
     n, steps = 200, 4
     ts = action_stream(n, steps, seed=9).astype(np.float32)
     dev = native.DeviceBuffer.from_array(ts)
@@ -182,7 +183,36 @@ def test_leg_form_matches_lane_form(native, form, tile):
     assert all(leg.world_error(i) == 0 for i in range(n))
 
 
-@pytest.mark.parametrize("form,persistent", [("leg", False), ("leg", True), ("pair", False), ("pair", True)])
+def test_shared_roots_form_block_sizes(native):
+    """The shared-root form with blocks of 128 threads (32 worlds) and 256
+    (64 worlds) against one world per lane, 200 worlds (partial last blocks),
+    bit for bit; launch shapes it cannot run are refused."""
+    n, steps = 200, 3
+    ts = action_stream(n, steps, seed=13).astype(np.float32)
+    dev = native.DeviceBuffer.from_array(ts)
+    per = n * 12 * 4
+    lane = _sim(native, n)
+    sims = []
+    for block in (128, 256):
+        g = _sim(native, n)
+        g.set_form("shared")
+        g.set_launch(block, 0)
+        sims.append(g)
+    with pytest.raises(ValueError):
+        sims[0].set_launch(64, 0)
+    with pytest.raises(ValueError):
+        sims[0].set_persistent(True)
+    for k in range(steps):
+        lane.step_device(dev.ptr + k * per)
+        lane.sync()
+        for g in sims:
+            g.step_device(dev.ptr + k * per)
+            g.sync()
+            assert np.array_equal(_bits(g, n), _bits(lane, n)), f"step {k}"
+
+
+@pytest.mark.parametrize("form,persistent", [("leg", False), ("leg", True), ("pair", False), ("pair", True),
+                                             ("shared", False)])
 def test_leg_form_batches_and_graphs(native, form, persistent):
     n = 256
     batches = [([0, 64], [32, 100]), ([5], [251]), ([0, 128, 200], [10, 60, 56])]

@@ -1,9 +1,9 @@
 """Edge cases of the step path on the GPU, through the C ABI.
 
 - world counts that fill no whole wavefront or quad group (1, 3, 17, 100):
-  every form (lane, four lanes per world, two lanes per world, one world per
-  wavefront) against the FP32 oracle after one step, and bit-identical to each
-  other;
+  every form (lane, four lanes per world, two lanes per world, four lanes per
+  world with shared root solves, one world per wavefront) against the FP32
+  oracle after one step, and bit-identical to each other;
 - reset inputs: an empty list (nothing changes), repeated ids (the last entry
   wins, as resetting in list order does), ids out of range (skipped, the
   others reset, the error reported);
@@ -48,7 +48,7 @@ def test_ragged_world_counts_all_forms(native, n):
     o.step(t)
     b = _states(o, n)
     ref = None
-    for form in ("lane", "leg", "pair", "wave"):
+    for form in ("lane", "leg", "pair", "shared", "wave"):
         g = native.Sim(n)
         g.set_form(form)
         g.reset(ids, seeds, diff)
@@ -98,7 +98,7 @@ def test_largest_batch_equals_full_step(native):
     ids = np.arange(n, dtype=np.int32)
     diff = P.difficulties(3, n)
     t = action_stream(n, 1)[0].astype(np.float32)
-    for form in ("lane", "leg"):
+    for form in ("lane", "leg", "shared"):
         g = native.Sim(n)
         h = native.Sim(n)
         for s in (g, h):
@@ -137,3 +137,4 @@ def test_full_size_forms_agree_and_repeat(native):
     assert _run_full(native, "leg", 4) == ref
     assert _run_full(native, "lane", 4) == ref
     assert _run_full(native, "pair", 4) == ref
+    assert _run_full(native, "shared", 4) == ref
