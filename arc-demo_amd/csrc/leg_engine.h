@@ -193,9 +193,11 @@ template <int K> struct RootsLocal {
  * four roots of world r in joint order, and every lane takes the torso and its
  * coxa back.  The roots then use all lanes of one wavefront instead of one lane
  * in four of every wavefront.  xs: [field][WPB] floats, RS_FIELDS fields. */
-constexpr int RS_FIELDS = 88;
+constexpr int RS_FIELDS = 88, RS_VFIELDS = 28; /* the pose and the velocity exchange */
 template <int WPB> struct RootsShared {
-    float *xs;
+    float *xs; /* pose exchange: torso, coxas (x, q, R), targets, torque sums */
+    float *xv; /* velocity exchange: torso (w, R), coxas (w), torque sums; a buffer of its own, so that
+                  each exchange needs no barrier before it is written (the other's barriers order it) */
     int g;        /* the lane's world slot in the block */
     int turn = 0; /* the wavefront whose first WPB lanes solve the roots: turns by substep, spreading the
                      chains over the SIMDs */
@@ -218,25 +220,10 @@ template <int WPB> struct RootsShared {
         for (int e = 0; e < 9; ++e)
             b.R.e[e / 3][e % 3] = at(f + 7 + e, w);
     }
-    __device__ __forceinline__ void put_vel(int f, int w, const Body &b, bool with_r) const
-    {
-        at(f + 0, w) = b.w.x; at(f + 1, w) = b.w.y; at(f + 2, w) = b.w.z;
-        if (with_r)
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                at(f + 7 + e, w) = b.R.e[e / 3][e % 3];
-    }
-    __device__ __forceinline__ void get_vel(int f, int w, Body &b, bool with_r) const
-    {
-        b.w = v3(at(f + 0, w), at(f + 1, w), at(f + 2, w));
-        if (with_r)
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                b.R.e[e / 3][e % 3] = at(f + 7 + e, w);
-    }
+    static constexpr int V_T = 0, V_C = 12, V_TA = 24; /* torso w, R; coxa w; torque sums */
+    __device__ __forceinline__ float &atv(int f, int w) const { return xv[f * WPB + w]; }
     __device__ __forceinline__ void positions(LegWorld<1> &W, int q, const Model &M)
     {
-        __syncthreads(); /* the previous exchange has been read */
         put_pose(F_C + 16 * q, g, W.b[0][0]);
         if (q == 0)
             put_pose(F_T, g, W.T);
@@ -266,33 +253,41 @@ template <int WPB> struct RootsShared {
     }
     __device__ __forceinline__ void velocities(LegWorld<1> &W, int q, const Model &M)
     {
-        __syncthreads();
-        put_vel(F_C + 16 * q, g, W.b[0][0], false);
-        if (q == 0)
-            put_vel(F_T, g, W.T, true);
-        at(F_TA + q, g) = W.tacc[0];
+        atv(V_C + 3 * q + 0, g) = W.b[0][0].w.x;
+        atv(V_C + 3 * q + 1, g) = W.b[0][0].w.y;
+        atv(V_C + 3 * q + 2, g) = W.b[0][0].w.z;
+        if (q == 0) {
+            atv(V_T + 0, g) = W.T.w.x; atv(V_T + 1, g) = W.T.w.y; atv(V_T + 2, g) = W.T.w.z;
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                atv(V_T + 3 + e, g) = W.T.R.e[e / 3][e % 3];
+        }
+        atv(V_TA + q, g) = W.tacc[0];
         __syncthreads();
         if (solver()) {
             const int r = threadIdx.x % 64;
             Body T;
-            get_vel(F_T, r, T, true);
+            T.w = v3(atv(V_T + 0, r), atv(V_T + 1, r), atv(V_T + 2, r));
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                T.R.e[e / 3][e % 3] = atv(V_T + 3 + e, r);
             static_for<0, LEGS>([&](auto t) {
                 Body C;
-                get_vel(F_C + 16 * t, r, C, false);
-                float ta = at(F_TA + t, r);
+                C.w = v3(atv(V_C + 3 * t + 0, r), atv(V_C + 3 * t + 1, r), atv(V_C + 3 * t + 2, r));
+                float ta = atv(V_TA + t, r);
                 ROBOT_JOINT(3 * t, M);
                 joint_velocities<KISO, KISO>(&T, &robot::body[0], C, robot::body[1], J, M.h, ta);
-                at(F_C + 16 * t + 0, r) = C.w.x;
-                at(F_C + 16 * t + 1, r) = C.w.y;
-                at(F_C + 16 * t + 2, r) = C.w.z;
-                at(F_TA + t, r) = ta;
+                atv(V_C + 3 * t + 0, r) = C.w.x;
+                atv(V_C + 3 * t + 1, r) = C.w.y;
+                atv(V_C + 3 * t + 2, r) = C.w.z;
+                atv(V_TA + t, r) = ta;
             });
-            at(F_T + 0, r) = T.w.x; at(F_T + 1, r) = T.w.y; at(F_T + 2, r) = T.w.z;
+            atv(V_T + 0, r) = T.w.x; atv(V_T + 1, r) = T.w.y; atv(V_T + 2, r) = T.w.z;
         }
         __syncthreads();
-        W.T.w = v3(at(F_T + 0, g), at(F_T + 1, g), at(F_T + 2, g));
-        W.b[0][0].w = v3(at(F_C + 16 * q + 0, g), at(F_C + 16 * q + 1, g), at(F_C + 16 * q + 2, g));
-        W.tacc[0] = at(F_TA + q, g);
+        W.T.w = v3(atv(V_T + 0, g), atv(V_T + 1, g), atv(V_T + 2, g));
+        W.b[0][0].w = v3(atv(V_C + 3 * q + 0, g), atv(V_C + 3 * q + 1, g), atv(V_C + 3 * q + 2, g));
+        W.tacc[0] = atv(V_TA + q, g);
         turn = (turn + 1) % (4 * WPB / 64); /* the block's wavefronts */
     }
 };

@@ -444,12 +444,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
     __shared__ float s_pose[4 * 7 * BLOCK];
     __shared__ int8_t s_tile[16 * 16 * WPB];
     __shared__ float s_roots[RS_FIELDS * WPB];
+    __shared__ float s_rootv[RS_VFIELDS * WPB];
     const Model &M = *m;
     LdsPoses<BLOCK> P0;
     P0.p = s_pose;
     P0.tid = threadIdx.x;
     RootsShared<WPB> roots;
     roots.xs = s_roots;
+    roots.xv = s_rootv;
     roots.g = threadIdx.x >> 2;
     const int n_ranges = batch ? batch->n_ranges : 0;
     if (batch)
