@@ -106,6 +106,32 @@ def kernel_pass(steps: int, warm: int) -> dict:
     return {"launch": cfg, "kernel_ms_avg": 1e3 * avg_s, "launches": steps}
 
 
+def forms_pass(steps: int, warm: int) -> dict:
+    """the step kernel of each form at 65,536 worlds from the same settled
+    state (all forms compute the same bits): HIP events around each launch"""
+    from arc import native
+    shapes = {"lane": 64, "leg": 64, "pair": 64, "shared": 256}
+    out = {}
+    for form, block in shapes.items():
+        s = new_sim(N)
+        t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+        per = N * NJ
+        s.set_form("leg")
+        for k in range(warm):
+            s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+        s.sync()
+        s.set_form(form)
+        s.set_launch(block, 0)
+        s.set_layout(16)
+        kms = 0.0
+        for k in range(steps):
+            m, _ = s.time_steps(t.ptr + 4 * per * (k % 8), 1, per)
+            kms += m
+        out[form] = {"launch": s.launch(), "kernel_ms_avg": kms / steps}
+        s.close()
+    return out
+
+
 def batched_pass(steps: int, warm: int) -> dict:
     from arc import native
     s = new_sim(N)
@@ -405,7 +431,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--latency-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "kernel", "rollout"], default="all")
+    ap.add_argument("--only", choices=["all", "kernel", "rollout", "forms"], default="all")
     ap.add_argument("--no-rollout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -420,6 +446,9 @@ def main() -> int:
 
     if args.only == "rollout":
         print(json.dumps({"rollout": rollout_pass()}), flush=True)
+        return 0
+    if args.only == "forms":
+        print(json.dumps({"forms_pass": forms_pass(steps, warm)}), flush=True)
         return 0
     if args.only == "kernel":
         # a plain 1 GiB device copy first: the counters' reading for known traffic
