@@ -386,6 +386,61 @@ def traffic_from_profile(kernel: str):
     return float(m.group(1)) if m else None
 
 
+def _ref_lib():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as P
+    flags = open("/proc/cpuinfo").read()
+    kind = "r64_v4" if all(f in flags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd")) else "r64_v3"
+    if not P.available(kind):
+        kind = "r64_v3" if P.available("r64_v3") else None
+    return P, kind
+
+
+def cpu_end_to_end(n: int, seconds: float) -> dict:
+    """The reference's own step loop with its fall rules and auto-reset
+    (arc/task.py:196-203 and arc/env.py:108-116 restated in numpy over the
+    reference engine, oracle/_ref, shipped flags): step, observe, falls,
+    reset of the fallen worlds; the host-side counterpart of the served path."""
+    P, kind = _ref_lib()
+    if kind is None:
+        return {"value": None, "unit": "world-steps/s", "note": "oracle/_ref not built"}
+    threads = min(16, P.cpu_threads())
+    s = P.Lib(kind).sim(n)
+    d = unit24(rng(1, n))
+    ids = np.arange(n, dtype=np.int32)
+    s.reset(ids, ids.astype(np.uint32), d)
+    defaults = s.robot_info()[0]
+    rs = rng(0, n * NJ * 16)
+    tg = [actions_to_targets(defaults, rs[j * n * NJ:(j + 1) * n * NJ]).astype(np.float64).reshape(n, NJ)
+          for j in range(16)]
+    half = 0.5 * 64 * 0.4
+    torso = np.zeros(n, dtype=np.int64)
+    episode = np.zeros(n, dtype=np.int64)
+    done, k, resets = 0, 0, 0
+    t0 = time.perf_counter()
+    while True:
+        s.step(tg[k % 16], threads)
+        root, rot, _, contacts, flags = s.observe()
+        torso = np.where(contacts[:, 4] > 0, torso + 1, 0)
+        bad = (flags != 0) | ~np.all(np.isfinite(root), axis=1)
+        void = np.any(np.abs(root[:, :2]) > half, axis=1) | (root[:, 2] < -2.0)
+        fell = bad | void | (rot[:, 2, 2] < 0.3) | (torso > 30)
+        ended = np.flatnonzero(fell)
+        if len(ended):
+            episode[ended] += 1
+            s.reset(ended.astype(np.int32), (ended + n * episode[ended]).astype(np.uint32), d[ended])
+            torso[ended] = 0
+            resets += len(ended)
+        done += n
+        k += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"{n} worlds x {k} steps ({el:.1f} s): arc_sim_step, arc_sim_observe, numpy fall rules, "
+                      f"arc_sim_reset of the fallen ({resets} resets)", "build": kind}
+
+
 def cpu_baseline(n: int, seconds: float) -> dict:
     """Reference engine (oracle/_ref, shipped flags) on this host's cores."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -519,6 +574,8 @@ def main() -> int:
         log("cpu baseline")
         cpu = cpu_baseline(args.cpu_worlds, args.cpu_seconds)
         line["cpu_baseline"] = cpu
+        log("cpu end-to-end (falls and resets)")
+        line["cpu_end_to_end"] = cpu_end_to_end(args.cpu_worlds, min(args.cpu_seconds, 5.0))
         if cpu.get("value"):
             line["ratios"] = {"served_vs_cpu": served["value"] / cpu["value"],
                               "served_vs_plain": served["value"] / plain["value"],
