@@ -102,6 +102,8 @@ def _task_lib():
             ("arct_set_levels", _int, [_vp, _vp]),
             ("arct_last_resets", _int, [_vp, _vp, _vp]),
             ("arct_seed_numpy", _int, [_vp, _vp, _int]),
+            ("arct_wait_stream", _int, [_vp, _vp]),
+            ("arct_signal_stream", _int, [_vp, _vp]),
             ("arct_numpy_state", _int, [_vp, _vp]),
         ]:
             fn = getattr(lib, name)
@@ -264,6 +266,22 @@ class DeviceTask:
         return {"bit_generator": "PCG64",
                 "state": {"state": int(out[0]) << 64 | int(out[1]), "inc": int(out[2]) << 64 | int(out[3])},
                 "has_uint32": int(out[4]), "uinteger": int(out[5])}
+
+    @staticmethod
+    def _stream(stream) -> int:
+        return int(getattr(stream, "cuda_stream", stream))
+
+    def wait_stream(self, stream) -> None:
+        """the task's next work waits, on the device, for the work queued so far
+        on `stream` (a torch.cuda.Stream or a HIP stream handle)"""
+        if self._lib.arct_wait_stream(self._ptr, self._stream(stream)):
+            self.sim._check()
+
+    def signal_stream(self, stream) -> None:
+        """work queued on `stream` from now on waits, on the device, for the
+        task's queued work"""
+        if self._lib.arct_signal_stream(self._ptr, self._stream(stream)):
+            self.sim._check()
 
     def sync(self) -> None:
         self._lib.arct_sync(self._ptr)

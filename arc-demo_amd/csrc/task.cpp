@@ -26,6 +26,7 @@ struct arc_task {
     hipGraphExec_t exec = nullptr;
     const float *graph_actions = nullptr;
     int graph_key[6] = { 0, 0, 0, 0, 0, 0 };
+    hipEvent_t ev_in = nullptr, ev_out = nullptr; /* ordering against a caller's stream */
 };
 
 extern "C" {
@@ -72,6 +73,10 @@ void arct_destroy(arc_task *t)
         (void)hipGraphExecDestroy(t->exec);
     if (t->graph)
         (void)hipGraphDestroy(t->graph);
+    if (t->ev_in)
+        (void)hipEventDestroy(t->ev_in);
+    if (t->ev_out)
+        (void)hipEventDestroy(t->ev_out);
     if (t->mem)
         (void)hipFree(t->mem);
     if (t->np_mem)
@@ -301,6 +306,35 @@ int arct_sync(arc_task *t)
     arc_sim *s = t->sim;
     HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arct_sync", -1);
     return s->err;
+}
+
+/* device-side ordering with a caller's stream (events, no host wait) */
+static hipError_t task_events(arc_task *t)
+{
+    hipError_t e = hipSuccess;
+    if (!t->ev_in)
+        e = hipEventCreateWithFlags(&t->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess && !t->ev_out)
+        e = hipEventCreateWithFlags(&t->ev_out, hipEventDisableTiming);
+    return e;
+}
+
+int arct_wait_stream(arc_task *t, void *stream)
+{
+    arc_sim *s = t->sim;
+    HIPCHK_RET(s, task_events(t), "arct_wait_stream", -1);
+    HIPCHK_RET(s, hipEventRecord(t->ev_in, (hipStream_t)stream), "arct_wait_stream", -1);
+    HIPCHK_RET(s, hipStreamWaitEvent(s->stream, t->ev_in, 0), "arct_wait_stream", -1);
+    return 0;
+}
+
+int arct_signal_stream(arc_task *t, void *stream)
+{
+    arc_sim *s = t->sim;
+    HIPCHK_RET(s, task_events(t), "arct_signal_stream", -1);
+    HIPCHK_RET(s, hipEventRecord(t->ev_out, s->stream), "arct_signal_stream", -1);
+    HIPCHK_RET(s, hipStreamWaitEvent((hipStream_t)stream, t->ev_out, 0), "arct_signal_stream", -1);
+    return 0;
 }
 
 int arct_targets(const arc_task *ct, double *out3)

@@ -312,16 +312,21 @@ def rollout_pass(steps: int = 64, rollouts: int = 3, n: int = 8192) -> dict:
     2 x 256 tanh MLP in PyTorch reads the device task's observations and
     writes its actions, both exchanged zero-copy (DLPack out, device pointer
     in); rewards and dones gathered into a rollout buffer on the GPU.  Runs in
-    its own process (torch's HIP runtime must load before libarcsim).  Timed
-    with per-world draw streams, then again with the draws from one numpy
+    its own process (torch's HIP runtime must load before libarcsim).  The
+    policy and the task are ordered on the device (HIP events between torch's
+    stream and the task's, no host wait inside a rollout); timed again with a
+    host synchronisation at each hand-over, and with the draws from one numpy
     generator in the reference's order."""
-    out = rollout_task(steps, rollouts, n, "world")
-    npm = rollout_task(steps, rollouts, n, "numpy")
-    out["numpy_stream"] = {k: npm[k] for k in ("value", "ms_per_step", "mean_reward", "done_fraction")}
+    out = rollout_task(steps, rollouts, n, "world", "stream")
+    host = rollout_task(steps, rollouts, n, "world", "host")
+    npm = rollout_task(steps, rollouts, n, "numpy", "stream")
+    keys = ("value", "ms_per_step", "mean_reward", "done_fraction")
+    out["host_synchronised"] = {k: host[k] for k in keys}
+    out["numpy_stream"] = {k: npm[k] for k in keys}
     return out
 
 
-def rollout_task(steps: int, rollouts: int, n: int, mode: str) -> dict:
+def rollout_task(steps: int, rollouts: int, n: int, mode: str, ordering: str) -> dict:
     import torch  # noqa: F401  (first: libarcsim then binds torch's HIP runtime)
     from arc import devtask
     cfg = devtask.TaskConfig(curriculum=False)
@@ -339,16 +344,23 @@ def rollout_task(steps: int, rollouts: int, n: int, mode: str) -> dict:
     buf_obs = torch.empty((steps, n, t.obs_size), device=dev)
     buf_rew = torch.empty((steps, n), device=dev)
     buf_done = torch.empty((steps, n), dtype=torch.bool, device=dev)
+    act = torch.zeros((n, 12), device=dev)  # one actions buffer: the task's captured graph reads it in place
     cur = torch.cuda.current_stream()
+    t.signal_stream(cur)                       # torch's work waits for the reset
 
     def rollout():
         with torch.no_grad():
             for k in range(steps):
                 buf_obs[k].copy_(obs)
-                act = policy(obs)
-                cur.synchronize()                  # actions complete before the task reads them
-                t.step(act, graph=True)
-                t.sync()                           # observations complete before torch reads them
+                act.copy_(policy(obs))
+                if ordering == "stream":
+                    t.wait_stream(cur)         # the task waits for the actions, on the device
+                    t.step(act, graph=True)
+                    t.signal_stream(cur)       # torch waits for the observations, on the device
+                else:
+                    cur.synchronize()          # actions complete before the task reads them
+                    t.step(act, graph=True)
+                    t.sync()                   # observations complete before torch reads them
                 buf_rew[k].copy_(rew)
                 buf_done[k].copy_((done_a | done_b).bool())
         cur.synchronize()
@@ -363,7 +375,8 @@ def rollout_task(steps: int, rollouts: int, n: int, mode: str) -> dict:
             "rollouts": rollouts, "ms_per_step": 1e3 * el / (steps * rollouts), "launch": cfgl,
             "obs_size": t.obs_size, "mean_reward": float(buf_rew.mean()), "done_fraction": float(buf_done.float().mean()),
             "policy": "random-init MLP 195-256-256-12 tanh, fp32", "exchange": "DLPack (obs, rewards, dones), device pointer (actions)",
-            "draws": mode}
+            "draws": mode, "ordering": "HIP events between torch's stream and the task's" if ordering == "stream"
+            else "host synchronisation at each hand-over"}
 
 
 def rollout_child() -> dict:

@@ -77,6 +77,16 @@ int arct_reset_all(arc_task *t);
 int arct_step(arc_task *t, const float *actions_dev, int graph);
 int arct_sync(arc_task *t);
 
+/* Ordering with a caller's HIP stream of the same device (for example
+ * torch.cuda.current_stream().cuda_stream) on the device, without a host
+ * wait: after arct_wait_stream, the task's next work starts once the work
+ * queued on `stream` so far has finished (the policy that wrote the
+ * actions); after arct_signal_stream, work queued on `stream` later starts
+ * once the task's queued work has finished (readers of the observations).
+ * No counterpart in the reference, whose task runs on the host. */
+int arct_wait_stream(arc_task *t, void *stream);
+int arct_signal_stream(arc_task *t, void *stream);
+
 /* Draw checkpoints and reset seeds from np.random.default_rng(seed), as the
  * reference's ArcTask (arc/task.py:93) and ArcVecEnv.reset (arc/env.py:93-95)
  * seed it; `words` is the seed as little-endian 32-bit words (numpy's

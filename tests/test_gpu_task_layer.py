@@ -141,3 +141,41 @@ def test_numpy_stream_reseed_and_border_rounds():
     o.reset(np.arange(n))
     assert np.array_equal(t.last_resets()[1], o.last_seeds) and _same_state(t, o.rng)
     t.close()
+
+
+def test_stream_ordering_matches_host_sync():
+    """A policy loop ordered on the device (arct_wait_stream / signal_stream,
+    no host wait) gives the same observations and rewards as one that
+    synchronises the host at each hand-over."""
+    import torch
+    from arc import devtask
+    n, steps = 256, 12
+    cfg = devtask.TaskConfig(max_steps=9, checkpoint_radius=3.0, fov=120.0, beached_steps=2, flip_up_z=0.95)
+    torch.manual_seed(0)
+    policy = torch.nn.Sequential(torch.nn.Linear(195, 64), torch.nn.Tanh(), torch.nn.Linear(64, 12)).cuda()
+    results = []
+    for ordering in ("host", "stream"):
+        t = devtask.DeviceTask(n, cfg, seed=4, form="leg")
+        t.reset()
+        t.sync()
+        obs, rew = torch.from_dlpack(t.obs), torch.from_dlpack(t.reward)
+        act = torch.zeros((n, 12), device="cuda")
+        cur = torch.cuda.current_stream()
+        rews = []
+        with torch.no_grad():
+            for k in range(steps):
+                act.copy_(2.0 * policy(obs))
+                if ordering == "stream":
+                    t.wait_stream(cur)
+                    t.step(act, graph=True)
+                    t.signal_stream(cur)
+                else:
+                    cur.synchronize()
+                    t.step(act, graph=True)
+                    t.sync()
+                rews.append(rew.clone())
+        cur.synchronize()
+        results.append((obs.cpu().numpy().copy(), torch.stack(rews).cpu().numpy(), t.last_resets()[0]))
+        t.close()
+    (o1, r1, e1), (o2, r2, e2) = results
+    assert np.array_equal(o1, o2) and np.array_equal(r1, r2) and np.array_equal(e1, e2)
