@@ -291,14 +291,19 @@ __global__ void k_task_after_reset(DevWorlds d, const Model *__restrict__ m, Tas
 
 /* the observation vector of world i (arc/task.py:250-280): gravity, body
  * velocities, joint state, previous action, checkpoint in the heading frame,
- * terrain heights around the torso, foot contacts; float32, non-finite -> 0 */
+ * terrain heights around the torso, foot contacts; float32, non-finite -> 0.
+ * OBS_GROUP threads per world: the group splits the height samples (each
+ * thread computes the torso's yaw frame itself, the same bits in every one),
+ * its first thread writes the other 52 values. */
+constexpr int OBS_GROUP = 16;
 __global__ void k_task_observe(DevWorlds d, const Model *__restrict__ m, TaskDev td, TaskCfg c,
                                const double *__restrict__ offsets, int k_off, float *__restrict__ obs,
                                const int *__restrict__ only, const int *__restrict__ count)
 {
     const Model &M = *m;
     const size_t S = d.stride;
-    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tg = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t0 = tg / OBS_GROUP, g = tg % OBS_GROUP;
     size_t i;
     if (only) {
         if (t0 >= *count)
@@ -309,24 +314,44 @@ __global__ void k_task_observe(DevWorlds d, const Model *__restrict__ m, TaskDev
             return;
         i = (size_t)t0;
     }
-    TaskView t;
-    task_view(d, M, i, t);
     const int size = 3 + 3 + 3 + 3 * NJ + 3 + k_off + LEGS;
     float *o = obs + i * size;
+    auto put = [&](int p, double v) { float f = (float)v; o[p] = isfinite(f) ? f : 0.0f; };
+    const int hp = 3 + 3 + 3 + 3 * NJ + 3; /* the first height's position */
+    {
+        /* heights as arc_sim_heights (csrc/arc_sim.c:185-203): binary32 yaw frame */
+        Q4 q = q4(d.q[0 * S + i], d.q[1 * S + i], d.q[2 * S + i], d.q[3 * S + i]);
+        V3 f3 = qrot(q, v3(1.0f, 0.0f, 0.0f));
+        float yaw = f_atan2(f3.y, f3.x), cy = f_cos(yaw), sy = f_sin(yaw);
+        float px = d.x[0 * S + i], py = d.x[1 * S + i];
+        const double pz = d.x[2 * S + i];
+        TerrainRef tr = terrain_ref(d, i);
+        tr.base = d.tbase[i];
+        tr.outside = d.outside[i];
+        for (int k = g; k < k_off; k += OBS_GROUP) {
+            float f = (float)offsets[2 * k], l = (float)offsets[2 * k + 1];
+            double h = terrain_height(M.terr, tr, px + cy * f - sy * l, py + sy * f + cy * l);
+            double r = pz - c.stand - h;
+            put(hp + k, r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r));
+        }
+    }
+    if (g != 0)
+        return;
+    TaskView t;
+    task_view(d, M, i, t);
     int p = 0;
-    auto put = [&](double v) { float f = (float)v; o[p++] = isfinite(f) ? f : 0.0f; };
     for (int a = 0; a < 3; ++a)
-        put(-t.R[2][a]);
+        put(p++, -t.R[2][a]);
     for (int a = 0; a < 3; ++a)
-        put(t.v[0] * t.R[0][a] + t.v[1] * t.R[1][a] + t.v[2] * t.R[2][a]);
+        put(p++, t.v[0] * t.R[0][a] + t.v[1] * t.R[1][a] + t.v[2] * t.R[2][a]);
     for (int a = 0; a < 3; ++a)
-        put(0.25 * (t.w[0] * t.R[0][a] + t.w[1] * t.R[1][a] + t.w[2] * t.R[2][a]));
+        put(p++, 0.25 * (t.w[0] * t.R[0][a] + t.w[1] * t.R[1][a] + t.w[2] * t.R[2][a]));
     for (int j = 0; j < NJ; ++j)
-        put(t.angle[j] - c.def[j]);
+        put(p++, t.angle[j] - c.def[j]);
     for (int j = 0; j < NJ; ++j)
-        put(0.1 * t.rate[j]);
+        put(p++, 0.1 * t.rate[j]);
     for (int j = 0; j < NJ; ++j)
-        put(td.prev_action[j * S + i]);
+        put(p++, td.prev_action[j * S + i]);
     {
         const double heading = atan2(t.R[1][0], t.R[0][0]);
         const double dx = td.target[0 * S + i] - t.pos[0], dy = td.target[1 * S + i] - t.pos[1];
@@ -334,28 +359,13 @@ __global__ void k_task_observe(DevWorlds d, const Model *__restrict__ m, TaskDev
         const double fwd = cs * dx + sn * dy, left = -sn * dx + cs * dy;
         double dist = hypot(fwd, left);
         dist = dist > 1e-6 ? dist : 1e-6;
-        put(fwd / dist);
-        put(left / dist);
-        put((dist < 6.0 ? dist : 6.0) / 6.0);
+        put(p++, fwd / dist);
+        put(p++, left / dist);
+        put(p++, (dist < 6.0 ? dist : 6.0) / 6.0);
     }
-    {
-        /* heights as arc_sim_heights (csrc/arc_sim.c:185-203): binary32 yaw frame */
-        Q4 q = q4(d.q[0 * S + i], d.q[1 * S + i], d.q[2 * S + i], d.q[3 * S + i]);
-        V3 f3 = qrot(q, v3(1.0f, 0.0f, 0.0f));
-        float yaw = f_atan2(f3.y, f3.x), cy = f_cos(yaw), sy = f_sin(yaw);
-        float px = d.x[0 * S + i], py = d.x[1 * S + i];
-        TerrainRef tr = terrain_ref(d, i);
-        tr.base = d.tbase[i];
-        tr.outside = d.outside[i];
-        for (int k = 0; k < k_off; ++k) {
-            float f = (float)offsets[2 * k], l = (float)offsets[2 * k + 1];
-            double h = terrain_height(M.terr, tr, px + cy * f - sy * l, py + sy * f + cy * l);
-            double r = t.pos[2] - c.stand - h;
-            put(r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r));
-        }
-    }
+    p += k_off;
     for (int l = 0; l < LEGS; ++l)
-        put(t.contact[l]);
+        put(p++, t.contact[l]);
 }
 
 __global__ void k_task_init(DevWorlds d, TaskDev td, uint32_t seed, double level)
@@ -619,7 +629,7 @@ hipError_t task_launch_np_place(const DevWorlds &d, const Model *m, const TaskDe
 hipError_t task_launch_observe(const DevWorlds &d, const Model *m, const TaskDev &td, const TaskCfg &c,
                                const double *offsets, int k_off, float *obs, bool listed, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_task_observe, dim3(blocks_for(d.n)), dim3(256), 0, s, d, m, td, c, offsets, k_off, obs,
+    hipLaunchKernelGGL(k_task_observe, dim3(blocks_for(d.n * OBS_GROUP)), dim3(256), 0, s, d, m, td, c, offsets, k_off, obs,
                        listed ? td.list : nullptr, listed ? td.count : nullptr);
     return hipGetLastError();
 }
