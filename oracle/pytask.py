@@ -10,6 +10,9 @@ device's numpy mode; without one it takes them from the caller (the device's
 per-world streams, read back by the test).  It restates everything else:
 observations, rewards, terminations and their precedence, the checkpoint
 rule's consequences (terrain height, distance, gaze, view), the curriculum.
+Pinned bit for bit, draws included, against the reference's own task module
+run over the reference engine (tests/test_task_golden.py,
+tests/golden/task_vectors.npz).
 """
 
 from __future__ import annotations

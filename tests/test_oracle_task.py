@@ -1,8 +1,6 @@
-"""Pins of the task-layer oracle (oracle/pytask.py), on the CPU.
-
-The reference's Python task cannot run here (its package imports gymnasium
-and stable-baselines3, absent, and requires Python 3.12), so the restatement
-is pinned by what the definitions fix: closed-form gaze angles, the
+"""Pins of the task-layer oracle (oracle/pytask.py), on the CPU, by what the
+definitions fix (tests/test_task_golden.py pins it further against vectors of
+the reference's own task module): closed-form gaze angles, the
 observation's layout and ranges on freshly reset worlds (upright torso,
 zero velocities, joint noise of at most 0.1 rad, unit heading vector),
 checkpoint geometry, the reward of a resting world, the termination
