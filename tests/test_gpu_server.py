@@ -93,3 +93,37 @@ def test_concurrent_clients(mods):
     assert st["world_steps"] == steps * sum(sizes)
     print(f"server: {st}")
     sv.close_server()
+
+
+def test_device_submission_matches_host(mods):
+    """Targets handed over in device memory (a libarcsim buffer or a torch
+    tensor) step the worlds exactly as the same targets handed over from host
+    memory, with both kinds in the same batches and several device requests
+    of one client in flight at once."""
+    import torch
+    native, server = mods
+    n, steps = 50, 3
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(200 * (per + 96))
+    a = sv.open(n, seed0=3, difficulty=0.7)
+    b = sv.open(n, seed0=3, difficulty=0.7)
+    acts = action_stream(n, steps, seed=9).astype(np.float32)
+    bufs = [native.DeviceBuffer.from_array(np.ascontiguousarray(acts[k])) for k in range(steps - 1)]
+    last = torch.from_numpy(np.ascontiguousarray(acts[steps - 1])).to("cuda")
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError):
+        sv.submit_device(b, last[:-1])
+    ta = [sv.submit(a, acts[k]) for k in range(steps)]
+    tb = [sv.submit_device(b, x) for x in bufs] + [sv.submit_device(b, last)]
+    for t in ta + tb:
+        assert sv.wait(t) > 0.0
+    ref = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    ref.reset(ids, (ids + 3).astype(np.uint32), np.full(n, 0.7))
+    for k in range(steps):
+        ref.step(acts[k].astype(np.float64))
+    root = ref.observe()[0]
+    assert np.array_equal(sv.observe(a), root)
+    assert np.array_equal(sv.observe(b), root)
+    assert sv.stats()["requests"] == 2 * steps
+    sv.close_server()
