@@ -183,6 +183,15 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         s->ts.seeds = s->ts.episode + N;
         s->ts.diff = (float *)(s->ts.seeds + N);
     }
+    /* default form: four lanes per world, the fastest untuned form at every
+     * world count measured (64 to 65,536); its contact lists are made here so
+     * that a failed allocation is a failed create, as in the reference */
+    if ((e = ensure_leg(s)) != hipSuccess) {
+        fprintf(stderr, "libarcsim: arc_sim_create: %s\n", hipGetErrorString(e));
+        arc_sim_destroy(s);
+        return nullptr;
+    }
+    s->form = 2;
     return s;
 }
 

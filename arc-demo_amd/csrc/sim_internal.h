@@ -39,8 +39,9 @@ struct arc_sim {
     int block, grid_blocks;
     int tile;              /* LDS terrain tile side of the lane form: 0 or 16 */
     int persistent;        /* lane form: wavefronts take work from a counter */
-    int form;              /* 0: one world per lane, 1: one world per wavefront, 2: four lanes per world,
-                              3: two lanes per world */
+    int form;              /* 0: one world per lane, 1: one world per wavefront, 2: four lanes per world
+                              (set at create), 3: two lanes per world, 4: four lanes per world with the
+                              hip joints of a block's worlds solved through LDS */
     WaveK wave;
     WaveK *d_wave;
     TaskK task;

@@ -74,10 +74,11 @@ int arcx_task_causes(const arc_sim *s, int *out);
 int arcx_task_stats(const arc_sim *s, long long *out5);
 int arcx_task_episodes(const arc_sim *s, unsigned int *out);
 
-/* Form of the step kernel: 0 = one world per lane (default; all substeps
- * of a control step in one launch, world state in registers), 1 = the plain
+/* Form of the step kernel: 0 = one world per lane (all substeps of a
+ * control step in one launch, world state in registers), 1 = the plain
  * form, one world per wavefront with the world staged in LDS, 2 = four lanes
- * per world (one leg each, torso replicated; shortest per-world latency),
+ * per world (one leg each, torso replicated; shortest per-world latency and
+ * the highest throughput untuned: the form of a new sim),
  * 3 = two lanes per world (two legs each), 4 = four lanes per world with the
  * four root joints of each block's worlds solved by the lanes of one wavefront
  * through LDS (blocks of 128 or 256 threads, no persistent distribution: form

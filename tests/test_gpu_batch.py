@@ -60,13 +60,15 @@ def test_batch_steps_only_its_ranges(native):
     assert np.all(np.abs(a - b) <= 1e-5 * (np.abs(b) + 1.0))
 
 
+@pytest.mark.parametrize("form", ["lane", "leg"])
 @pytest.mark.parametrize("block", [64, 128, 256])
-def test_launch_configurations_bit_identical(native, block):
+def test_launch_configurations_bit_identical(native, block, form):
     n, steps = 300, 2
     ts = action_stream(n, steps).astype(np.float32)
     dev = native.DeviceBuffer.from_array(ts)
     per = n * 12 * 4
     base = _sim(native, n)
+    base.set_form("lane")
     for k in range(steps):
         base.step_device(dev.ptr + k * per)
     base.sync()
@@ -74,10 +76,12 @@ def test_launch_configurations_bit_identical(native, block):
     for tile in (0, 16):
         for persistent in (False, True):
             g = _sim(native, n)
+            g.set_form(form)
             g.set_launch(block, 0)
             g.set_layout(tile)
             g.set_persistent(persistent)
-            assert g.launch()["block"] == block and g.launch()["tile"] == tile
+            # the group forms always stage the 16 x 16 tile; the layout switch is the lane form's
+            assert g.launch()["block"] == block and g.launch()["tile"] == (tile if form == "lane" else 16)
             for k in range(steps):
                 g.step_batch([0], [n], dev.ptr + k * per)
             g.sync()
@@ -170,6 +174,7 @@ def test_leg_form_matches_lane_form(native, form, tile):
     dev = native.DeviceBuffer.from_array(ts)
     per = n * 12 * 4
     lane, leg = _sim(native, n), _sim(native, n)
+    lane.set_form("lane")
     leg.set_form(form)
     leg.set_layout(tile)
     lane.set_layout(tile)

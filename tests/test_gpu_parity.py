@@ -160,6 +160,7 @@ def test_wave_form_matches_lane_form(native, difficulty):
     n = 96
     diff = P.difficulties(1, n) if difficulty == "mixed" else np.full(n, difficulty)
     lane, o = _pair(native, n, diff)
+    lane.set_form("lane")
     wave = native.Sim(n)
     wave.set_form("wave")
     ids = np.arange(n, dtype=np.int32)
