@@ -185,13 +185,17 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
     }
     /* default form: four lanes per world, the fastest untuned form at every
      * world count measured (64 to 65,536); its contact lists are made here so
-     * that a failed allocation is a failed create, as in the reference */
-    if ((e = ensure_leg(s)) != hipSuccess) {
+     * that a failed allocation is a failed create, as in the reference.  A
+     * robot whose contact windows exceed the group forms' cap keeps the
+     * one-world-per-lane form. */
+    e = ensure_leg(s);
+    if (e == hipSuccess) {
+        s->form = 2;
+    } else if (e != hipErrorNotSupported) {
         fprintf(stderr, "libarcsim: arc_sim_create: %s\n", hipGetErrorString(e));
         arc_sim_destroy(s);
         return nullptr;
     }
-    s->form = 2;
     return s;
 }
 

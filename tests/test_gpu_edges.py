@@ -1,5 +1,7 @@
 """Edge cases of the step path on the GPU, through the C ABI.
 
+- a sim of no worlds: create, step, observe, heights and a batch of no
+  requests do nothing and report no error (the reference's loops are empty);
 - world counts that fill no whole wavefront or quad group (1, 3, 17, 100):
   every form (lane, four lanes per world, two lanes per world, four lanes per
   world with shared root solves, one world per wavefront) against the FP32
@@ -35,6 +37,21 @@ def native():
 
 def _states(sim, n):
     return np.stack([sim.state(i) for i in range(n)])
+
+
+def test_no_worlds(native):
+    for form in ("lane", "leg", "shared"):
+        g = native.Sim(0)
+        g.set_form(form)
+        g.reset(np.zeros(0, np.int32), np.zeros(0, np.uint32), np.zeros(0))
+        g.step(np.zeros((0, 12)))
+        root, joints, contacts, flags = g.observe()
+        assert root.shape == (0, 13) and joints.shape == (0, 36) and flags.shape == (0,)
+        assert g.heights(np.zeros((3, 2))).shape == (0, 3)
+        buf = native.DeviceBuffer(64)
+        g.step_batch([], [], buf.ptr, task=True)
+        g.sync()
+        g.close()
 
 
 @pytest.mark.parametrize("n", [1, 3, 17, 100])
