@@ -352,14 +352,20 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
         return -1;
     }
     const size_t nt = (size_t)s->n * NJ;
-    std::vector<float> ht(nt);
+    /* default pose plus 0.6 x uniform actions, a fresh draw for each step of a
+     * timing run (set j from the action stream seeded j): the step's cost
+     * depends on the action law, and a law that repeats a few sets makes the
+     * legs oscillate and the contact load grow */
+    const int sets = std::min(steps, 16) + 1;
+    std::vector<float> ht(nt * (size_t)sets);
     {
-        /* default pose plus 0.6 x uniform actions from the action stream (seed 0) */
         std::vector<unsigned> u(nt);
-        arcx_rng_stream(0u, (long long)nt, u.data());
-        for (size_t k = 0; k < nt; ++k) {
-            double a = (double)(u[k] >> 8) * (1.0 / 8388608.0) - 1.0;
-            ht[k] = (float)((double)s->su.info.def[k % NJ] + 0.6 * a);
+        for (int j = 0; j < sets; ++j) {
+            arcx_rng_stream((unsigned)j, (long long)nt, u.data());
+            for (size_t k = 0; k < nt; ++k) {
+                double a = (double)(u[k] >> 8) * (1.0 / 8388608.0) - 1.0;
+                ht[(size_t)j * nt + k] = (float)((double)s->su.info.def[k % NJ] + 0.6 * a);
+            }
         }
     }
     float *dt = nullptr;
@@ -371,12 +377,12 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
     int best[5] = { keep_block, keep_tile, keep_grid, keep_pers, keep_form };
     double best_ms = 1e30;
     do {
-        if (hipMalloc((void **)&dt, nt * sizeof(float)) != hipSuccess || hipMalloc(&snap, s->dev.bytes) != hipSuccess
+        if (hipMalloc((void **)&dt, ht.size() * sizeof(float)) != hipSuccess || hipMalloc(&snap, s->dev.bytes) != hipSuccess
             || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
             set_error(s, -1, "arcx_autotune: allocation", hipSuccess);
             break;
         }
-        bool ok = hipMemcpyAsync(dt, ht.data(), nt * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
+        bool ok = hipMemcpyAsync(dt, ht.data(), ht.size() * sizeof(float), hipMemcpyHostToDevice, s->stream) == hipSuccess
                   && hipMemcpyAsync(snap, s->dev.base, s->dev.bytes, hipMemcpyDeviceToDevice, s->stream) == hipSuccess
                   && hipStreamSynchronize(s->stream) == hipSuccess;
         const int blocks[3] = { 64, 128, 256 }, tiles[2] = { 16, 0 }, forms[4] = { 0, 2, 3, 4 };
@@ -401,7 +407,7 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
                     ok = ok && arcx_step_batch(s, &start, &count, 1, dt, 0, 0) == 0;
                     ok = ok && hipEventRecord(e0, s->stream) == hipSuccess;
                     for (int k = 0; k < steps && ok; ++k)
-                        ok = arcx_step_batch(s, &start, &count, 1, dt, 0, 0) == 0;
+                        ok = arcx_step_batch(s, &start, &count, 1, dt + (size_t)(1 + k % (sets - 1)) * nt, 0, 0) == 0;
                     ok = ok && hipEventRecord(e1, s->stream) == hipSuccess;
                     ok = ok && hipEventSynchronize(e1) == hipSuccess;
                     ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;

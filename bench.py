@@ -26,8 +26,12 @@ Beside it, in the same run:
   cpu       the reference engine built from its sources with its shipped
             flags (oracle/_ref), on this host's cores, bounded sample.
 
-Prints ONE JSON line.  `--only kernel` runs just the kernel timing pass (for
-counter collection under rocprofv3).
+Every pass draws fresh actions for every step it takes (target_pool): the
+step kernel's cost depends on the action law.  Prints ONE JSON line.
+`--only kernel` runs just the kernel timing pass (for counter collection
+under rocprofv3); `--only forms` times every step form from one settled
+state; `--only actions` times the kernel under laws that repeat a few action
+sets against fresh actions.
 """
 
 from __future__ import annotations
@@ -71,6 +75,30 @@ def actions_to_targets(defaults: np.ndarray, u: np.ndarray) -> np.ndarray:
     return (np.tile(defaults, u.size // NJ) + 0.6 * a).astype(np.float32)
 
 
+MAX_SETS = 512
+
+
+def target_pool(defaults: np.ndarray, n: int, sets: int, seed: int = 0) -> np.ndarray:
+    """`sets` independent draws of the action law for n worlds, float32
+    (sets, n, 12); set k comes from the engine RNG seeded seed + k.  A pass
+    steps with set k at its k-th step, so every timed step gets actions of its
+    own: the step's cost depends on the action law (a law that cycles through
+    a few sets makes the legs oscillate and the contact load grow; see
+    --only actions)."""
+    out = np.empty((sets, n, NJ), dtype=np.float32)
+    d = np.asarray(defaults, dtype=np.float64)[None, :]
+    for k in range(sets):
+        # actions_to_targets, broadcast over the worlds instead of tiled
+        a = (rng(seed + k, n * NJ) >> 8).reshape(n, NJ).astype(np.float64) * (1.0 / 8388608.0) - 1.0
+        out[k] = d + 0.6 * a
+    return out
+
+
+def pool_sets(warm: int, steps: int) -> int:
+    """sets of a pass's pool: one per step of its warm-up and timed steps (at most MAX_SETS, then cycled)"""
+    return max(8, min(MAX_SETS, warm + steps))
+
+
 def new_sim(n: int, difficulty=None):
     from arc import native
     s = native.Sim(n)
@@ -80,12 +108,12 @@ def new_sim(n: int, difficulty=None):
     return s
 
 
-def warm_and_tune(s, n, t, warm):
-    """--warmup steps with resets (the contact load settles after ~80 steps),
-    then the autotuner on the settled worlds"""
+def warm_and_tune(s, n, t, warm, sets):
+    """--warmup steps with resets (the contact load settles after ~100 steps),
+    then the autotuner on the settled worlds; t holds `sets` target sets"""
     per = n * NJ
     for k in range(warm):
-        s.step_batch([0], [n], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+        s.step_batch([0], [n], t.ptr + 4 * per * (k % sets), task=True, graph=True)
     s.sync()
     cfg, _ = s.autotune(3)
     return cfg
@@ -95,12 +123,13 @@ def kernel_pass(steps: int, warm: int) -> dict:
     """the step kernel at 65,536 worlds: HIP events around each launch on the engine stream"""
     from arc import native
     s = new_sim(N)
-    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+    S = pool_sets(warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, S))
     per = N * NJ
-    cfg = warm_and_tune(s, N, t, warm)
+    cfg = warm_and_tune(s, N, t, warm, S)
     kms = 0.0
     for k in range(steps):
-        m, _ = s.time_steps(t.ptr + 4 * per * (k % 8), 1, per)
+        m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
         kms += m
     avg_s = kms / 1e3 / steps
     return {"launch": cfg, "kernel_ms_avg": 1e3 * avg_s, "launches": steps}
@@ -112,35 +141,61 @@ def forms_pass(steps: int, warm: int) -> dict:
     from arc import native
     shapes = {"lane": 64, "leg": 64, "pair": 64, "shared": 256}
     out = {}
+    S = pool_sets(warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(native.Sim(1).default_angles, N, S))
+    per = N * NJ
     for form, block in shapes.items():
         s = new_sim(N)
-        t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
-        per = N * NJ
         s.set_form("leg")
         for k in range(warm):
-            s.step_batch([0], [N], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+            s.step_batch([0], [N], t.ptr + 4 * per * (k % S), task=True, graph=True)
         s.sync()
         s.set_form(form)
         s.set_launch(block, 0)
         s.set_layout(16)
         kms = 0.0
         for k in range(steps):
-            m, _ = s.time_steps(t.ptr + 4 * per * (k % 8), 1, per)
+            m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
             kms += m
         out[form] = {"launch": s.launch(), "kernel_ms_avg": kms / steps}
         s.close()
+    t.close()
+    return out
+
+
+def action_law_pass(steps: int, warm: int) -> dict:
+    """the step kernel at 65,536 worlds under action laws that repeat with
+    period p (p sets of the law in turn) and with fresh actions at every step:
+    a new sim per law, --warmup steps of that law with resets, then --steps
+    launches timed with HIP events"""
+    from arc import native
+    S = pool_sets(warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(native.Sim(1).default_angles, N, S))
+    per = N * NJ
+    out = {}
+    for p in (2, 4, 8, 32, S):
+        s = new_sim(N)
+        for k in range(warm):
+            s.step_batch([0], [N], t.ptr + 4 * per * (k % p), task=True, graph=True)
+        s.sync()
+        kms = sum(s.time_steps(t.ptr + 4 * per * ((warm + k) % p), 1, per)[0] for k in range(steps))
+        out["fresh" if p == S else f"period_{p}"] = {"period": p, "kernel_ms_avg": kms / steps,
+                                                     "launch": s.launch()}
+        s.close()
+    t.close()
     return out
 
 
 def batched_pass(steps: int, warm: int) -> dict:
     from arc import native
     s = new_sim(N)
-    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
+    S = pool_sets(warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, S))
     per = N * NJ
-    cfg = warm_and_tune(s, N, t, warm)
+    cfg = warm_and_tune(s, N, t, warm, S)
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step_batch([0], [N], t.ptr + 4 * per * ((warm + k) % 8), task=True, graph=True)
+        s.step_batch([0], [N], t.ptr + 4 * per * ((warm + k) % S), task=True, graph=True)
     s.sync()
     el = time.perf_counter() - t0
     stats = s.task_stats()
@@ -153,14 +208,15 @@ def abi_pass(steps: int, warm: int) -> dict:
     step, return on completion (PCIe-inclusive; never the headline)"""
     from arc import native
     s = new_sim(N)
-    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 8)))
-    cfg = warm_and_tune(s, N, t, warm)
-    host = [actions_to_targets(s.default_angles, rng(50 + k, N * NJ)).astype(np.float64).reshape(N, NJ)
-            for k in range(4)]
-    s.step(host[0])
+    S = pool_sets(warm, 0)
+    hp = target_pool(s.default_angles, N, S + steps + 1)
+    t = native.DeviceBuffer.from_array(hp[:S])
+    cfg = warm_and_tune(s, N, t, warm, S)
+    host = [hp[S + k].astype(np.float64) for k in range(steps + 1)]
+    s.step(host[steps])
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step(host[k % 4])
+        s.step(host[k])
     el = time.perf_counter() - t0
     return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "launch": cfg,
             "path": "arc_sim_step(host float64 targets): convert, copy to HBM, step, synchronize"}
@@ -170,13 +226,13 @@ def plain_pass(steps: int = 3) -> dict:
     from arc import native
     s = new_sim(N)
     s.set_form("wave")
-    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, N * NJ * 2)))
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, steps + 1))
     per = N * NJ
     s.step_task(t.ptr)
     s.sync()
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step_task(t.ptr + 4 * per * ((k + 1) % 2))
+        s.step_task(t.ptr + 4 * per * (k + 1))
     s.sync()
     el = time.perf_counter() - t0
     return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
@@ -187,13 +243,14 @@ def latency_pass(steps: int, warm: int) -> dict:
     from arc import native
     n = 4096
     s = new_sim(n, 0.5)
-    t = native.DeviceBuffer.from_array(actions_to_targets(s.default_angles, rng(0, n * NJ * 8)))
+    S = pool_sets(warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, n, S))
     per = n * NJ
-    cfg = warm_and_tune(s, n, t, warm)
+    cfg = warm_and_tune(s, n, t, warm, S)
     lat = []
     for k in range(steps):
         t0 = time.perf_counter()
-        s.step_batch([0], [n], t.ptr + 4 * per * (k % 8), task=True, graph=True)
+        s.step_batch([0], [n], t.ptr + 4 * per * ((warm + k) % S), task=True, graph=True)
         s.sync()
         lat.append(1e3 * (time.perf_counter() - t0))
     lat = np.array(lat)
@@ -214,17 +271,19 @@ def served_pass(steps: int, warm: int) -> dict:
     sv = server.StepServer(N * per_world * 1.0001)
     n = N // CLIENTS
     cids = [sv.open(n, seed0=c * n, difficulty=-1.0) for c in range(CLIENTS)]
-    defaults = native.Sim(1).default_angles
-    acts = [[actions_to_targets(defaults, rng(1000 + 8 * c + j, n * NJ)) for j in range(4)] for c in range(CLIENTS)]
-    dev = [[native.DeviceBuffer.from_array(a) for a in row] for row in acts]
+    S = pool_sets(warm, steps)
+    hp = target_pool(native.Sim(1).default_angles, N, S, seed=1000)
+    dev = native.DeviceBuffer.from_array(hp)
 
     def run(k0, count, depth, device):
+        """client c's step k: rows [c n, (c + 1) n) of set k mod S"""
         inflight = []
         for k in range(k0, k0 + count):
+            j = k % S
             if device:
-                inflight.append([sv.submit_device(c, dev[c][k % 4]) for c in cids])
+                inflight.append([sv.submit_device(c, dev.ptr + 4 * NJ * (j * N + c * n)) for c in cids])
             else:
-                inflight.append([sv.submit(c, acts[c][k % 4]) for c in cids])
+                inflight.append([sv.submit(c, hp[j, c * n:(c + 1) * n]) for c in cids])
             if len(inflight) >= depth:
                 for x in inflight.pop(0):
                     sv.wait(x)
@@ -249,9 +308,7 @@ def served_pass(steps: int, warm: int) -> dict:
                      "targets": "device" if device else "host (PCIe-inclusive)",
                      "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "requests_per_batch": st["requests_per_batch"]}
     sv.close_server()
-    for row in dev:
-        for b in row:
-            b.close()
+    dev.close()
     out["value"] = out["pipelined"]["value"]
     out["ms_per_step"] = out["pipelined"]["ms_per_step"]
     return out
@@ -267,20 +324,27 @@ def serving_pass(seconds: float) -> dict:
     sv = server.StepServer(sum(sizes) * per_world * 1.0001)
     cids = [sv.open(n, seed0=10000 * c, difficulty=-1.0) for c, n in enumerate(sizes)]
     defaults = native.Sim(1).default_angles
-    acts = [actions_to_targets(defaults, rng(2000 + c, n * NJ)) for c, n in enumerate(sizes)]
+    pools = [target_pool(defaults, n, 64, seed=20000 + 100 * c) for c, n in enumerate(sizes)]
+    used = [0] * CLIENTS
+
+    def acts(c):
+        """client c's next request: its next set of 64, in turn"""
+        used[c] += 1
+        return pools[c][used[c] % 64]
+
     for _ in range(30):  # settle the worlds, then tune the launch on them
-        for x in [sv.submit(c, acts[c]) for c in cids]:
+        for x in [sv.submit(c, acts(c)) for c in cids]:
             sv.wait(x)
     launch, _ = sv.autotune(3)
     # saturation: every client keeps one request in flight
     stop = time.perf_counter() + min(1.0, seconds / 3)
     for c in cids:
-        sv.wait(sv.submit(c, acts[c]))
+        sv.wait(sv.submit(c, acts(c)))
     sv.reset_stats()
 
     def closed(c):
         while time.perf_counter() < stop:
-            sv.wait(sv.submit(c, acts[c]))
+            sv.wait(sv.submit(c, acts(c)))
 
     th = [threading.Thread(target=closed, args=(c,)) for c in cids]
     for x in th:
@@ -302,7 +366,7 @@ def serving_pass(seconds: float) -> dict:
         d = t0 + w - time.perf_counter()
         if d > 0:
             time.sleep(d)
-        tickets.append(sv.submit(cids[c], acts[c]))
+        tickets.append(sv.submit(cids[c], acts(c)))
     for x in tickets:
         sv.wait(x)
     st = sv.stats()
@@ -434,16 +498,15 @@ def cpu_end_to_end(n: int, seconds: float) -> dict:
     ids = np.arange(n, dtype=np.int32)
     s.reset(ids, ids.astype(np.uint32), d)
     defaults = s.robot_info()[0]
-    rs = rng(0, n * NJ * 16)
-    tg = [actions_to_targets(defaults, rs[j * n * NJ:(j + 1) * n * NJ]).astype(np.float64).reshape(n, NJ)
-          for j in range(16)]
     half = 0.5 * 64 * 0.4
     torso = np.zeros(n, dtype=np.int64)
     episode = np.zeros(n, dtype=np.int64)
-    done, k, resets = 0, 0, 0
-    t0 = time.perf_counter()
-    while True:
-        s.step(tg[k % 16], threads)
+    done, k, resets, el = 0, 0, 0, 0.0
+    while el < seconds:
+        # fresh actions every step (drawn outside the timed work)
+        tg = actions_to_targets(defaults, rng(k, n * NJ)).astype(np.float64).reshape(n, NJ)
+        t0 = time.perf_counter()
+        s.step(tg, threads)
         root, rot, _, contacts, flags = s.observe()
         torso = np.where(contacts[:, 4] > 0, torso + 1, 0)
         bad = (flags != 0) | ~np.all(np.isfinite(root), axis=1)
@@ -455,14 +518,13 @@ def cpu_end_to_end(n: int, seconds: float) -> dict:
             s.reset(ended.astype(np.int32), (ended + n * episode[ended]).astype(np.uint32), d[ended])
             torso[ended] = 0
             resets += len(ended)
+        el += time.perf_counter() - t0
         done += n
         k += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    el = time.perf_counter() - t0
     return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
             "sample": f"{n} worlds x {k} steps ({el:.1f} s): arc_sim_step, arc_sim_observe, numpy fall rules, "
-                      f"arc_sim_reset of the fallen ({resets} resets)", "build": kind}
+                      f"arc_sim_reset of the fallen ({resets} resets); fresh uniform actions every step",
+            "build": kind}
 
 
 def cpu_baseline(n: int, seconds: float) -> dict:
@@ -482,22 +544,19 @@ def cpu_baseline(n: int, seconds: float) -> dict:
     ids = np.arange(n, dtype=np.int32)
     s.reset(ids, ids.astype(np.uint32), d)
     defaults = s.robot_info()[0]
-    rs = rng(0, n * NJ * 16)
-    tg = [actions_to_targets(defaults, rs[j * n * NJ:(j + 1) * n * NJ]).astype(np.float64).reshape(n, NJ)
-          for j in range(16)]
-    done, k = 0, 0
-    t0 = time.perf_counter()
-    while True:
-        s.step(tg[k % 16], threads)
+    done, k, el = 0, 0, 0.0
+    while el < seconds:
+        # fresh actions every step (drawn outside the timed work)
+        tg = actions_to_targets(defaults, rng(k, n * NJ)).astype(np.float64).reshape(n, NJ)
+        t0 = time.perf_counter()
+        s.step(tg, threads)
+        el += time.perf_counter() - t0
         done += n
         k += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    el = time.perf_counter() - t0
     cpu = flags.split("model name")[1].split("\n")[0].strip(": \t") if "model name" in flags else "?"
     return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"{n} worlds x {k} steps ({el:.1f} s), per-world difficulty U[0,1), same action law, "
-                      "arc_sim_step only", "build": kind, "cpu": cpu}
+            "sample": f"{n} worlds x {k} steps ({el:.1f} s of stepping), per-world difficulty U[0,1), fresh "
+                      "uniform actions every step (the GPU passes' law), arc_sim_step only", "build": kind, "cpu": cpu}
 
 
 def main() -> int:
@@ -510,7 +569,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--latency-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "kernel", "rollout", "forms"], default="all")
+    ap.add_argument("--only", choices=["all", "kernel", "rollout", "forms", "actions"], default="all")
     ap.add_argument("--no-rollout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -528,6 +587,9 @@ def main() -> int:
         return 0
     if args.only == "forms":
         print(json.dumps({"forms_pass": forms_pass(steps, warm)}), flush=True)
+        return 0
+    if args.only == "actions":
+        print(json.dumps({"action_law_pass": action_law_pass(steps, warm)}), flush=True)
         return 0
     if args.only == "kernel":
         # a plain 1 GiB device copy first: the counters' reading for known traffic
