@@ -212,7 +212,8 @@ hipError_t prepare_launch(arc_sim *s)
     return hipSuccess;
 }
 
-/* every world of the sim, lane or leg form, in the configured distribution */
+/* every world of the sim in the configured register-resident form (lane, leg,
+ * pair or shared) and distribution */
 hipError_t launch_lane_all(arc_sim *s, const float *targets)
 {
     hipError_t e;
@@ -283,7 +284,8 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
             return -1;
         }
     if (s->form == 1) {
-        set_error(s, -1, "arcx_step_batch: the batched launch runs the lane or leg form", hipSuccess);
+        set_error(s, -1, "arcx_step_batch: the batched launch runs the register-resident forms, not the plain form",
+                  hipSuccess);
         return -1;
     }
     if (is_leg(s))
@@ -338,9 +340,9 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
 
 int arcx_graph_captures(const arc_sim *s) { return s->batch ? s->batch->captures : 0; }
 
-/* Times every launch configuration of the lane form on this handle's worlds
- * and keeps the fastest.  The device state is copied aside first and put
- * back afterwards, so the worlds end where they started. */
+/* Times every launch configuration of the register-resident forms on this
+ * handle's worlds and keeps the fastest.  The device state is copied aside
+ * first and put back afterwards, so the worlds end where they started. */
 int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
 {
     if (s->n <= 0 || steps <= 0) {
@@ -348,7 +350,8 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
         return -1;
     }
     if (s->form == 1) {
-        set_error(s, -1, "arcx_autotune: the tuner configures the lane and leg forms", hipSuccess);
+        set_error(s, -1, "arcx_autotune: the tuner configures the register-resident forms, not the plain form",
+                  hipSuccess);
         return -1;
     }
     const size_t nt = (size_t)s->n * NJ;

@@ -59,7 +59,8 @@ void set_error(const arc_sim *cs, int code, const char *what, hipError_t e);
 hipError_t launch_form(arc_sim *s, const float *targets);
 /* releases the batch state of a handle */
 void batch_free(arc_sim *s);
-/* every world of the sim, lane or leg form, in the configured distribution */
+/* every world of the sim in the configured register-resident form (lane, leg,
+ * pair or shared) and distribution */
 hipError_t launch_lane_all(arc_sim *s, const float *targets);
 /* the leg form's contact lists (allocated on first use) */
 hipError_t ensure_leg(arc_sim *s);
