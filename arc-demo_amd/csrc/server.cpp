@@ -36,6 +36,7 @@ using clock_type = std::chrono::steady_clock;
 struct Request {
     long long ticket;
     int client;
+    int start = 0, count = 0;     /* the client's worlds, copied when the batch is formed */
     std::vector<float> targets;   /* host targets (arcs_submit), or */
     const float *dev = nullptr;   /* the client's device targets (arcs_submit_device) */
     clock_type::time_point submitted;
@@ -144,6 +145,11 @@ static void dispatch_loop(arc_server *sv)
                 if (!c.queue.empty()) {
                     f.reqs.push_back(std::move(c.queue.front()));
                     c.queue.pop_front();
+                    /* the range as it is now: the client table may grow (and move) once
+                     * the lock is released, and a client's range is fixed while it has
+                     * requests queued or in flight */
+                    f.reqs.back().start = c.start;
+                    f.reqs.back().count = c.count;
                 }
             f.buf = (int)(sv->launched % DEPTH);
         }
@@ -154,8 +160,7 @@ static void dispatch_loop(arc_server *sv)
             arc_sim *s = sv->sim;
             float *h = sv->h_targets[f.buf], *d = sv->d_targets[f.buf];
             for (auto &r : f.reqs) {
-                const Client &c = sv->clients[r.client];
-                const size_t off = (size_t)c.start * NJ, bytes = (size_t)c.count * NJ * sizeof(float);
+                const size_t off = (size_t)r.start * NJ, bytes = (size_t)r.count * NJ * sizeof(float);
                 if (r.dev) {
                     f.ok = f.ok && hipMemcpyAsync(d + off, r.dev, bytes, hipMemcpyDeviceToDevice, sv->copy)
                                        == hipSuccess;
@@ -164,9 +169,9 @@ static void dispatch_loop(arc_server *sv)
                     f.ok = f.ok && hipMemcpyAsync(d + off, h + off, bytes, hipMemcpyHostToDevice, sv->copy)
                                        == hipSuccess;
                 }
-                starts.push_back(c.start);
-                counts.push_back(c.count);
-                f.worlds += c.count;
+                starts.push_back(r.start);
+                counts.push_back(r.count);
+                f.worlds += r.count;
             }
             f.ok = f.ok && hipEventRecord(sv->copied[f.buf], sv->copy) == hipSuccess
                    && hipStreamWaitEvent(s->stream, sv->copied[f.buf], 0) == hipSuccess;
@@ -332,8 +337,10 @@ int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficult
         sv->free_ranges.erase(std::remove_if(sv->free_ranges.begin(), sv->free_ranges.end(),
                                              [](const std::pair<int, int> &r) { return r.second == 0; }),
                               sv->free_ranges.end());
+        /* a slot is reused once its client is closed and its range released
+         * (count 0): a closing client keeps its slot until then */
         for (size_t k = 0; k < sv->clients.size(); ++k)
-            if (!sv->clients[k].open && sv->clients[k].queue.empty()) {
+            if (!sv->clients[k].open && sv->clients[k].queue.empty() && sv->clients[k].count == 0) {
                 id = (int)k;
                 break;
             }
@@ -381,16 +388,17 @@ int arcs_close(arc_server *sv, int client)
     std::unique_lock<std::mutex> lk(sv->mu);
     if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open)
         return -1;
-    Client &c = sv->clients[client];
-    c.open = false; /* no new submissions */
+    sv->clients[client].open = false; /* no new submissions */
+    /* the table is indexed afresh after the wait: arcs_open may grow it meanwhile */
     sv->done.wait(lk, [&] {
-        if (!c.queue.empty())
+        if (!sv->clients[client].queue.empty())
             return false;
         for (auto &kv : sv->inflight)
             if (kv.second == client)
                 return false;
         return true;
     });
+    Client &c = sv->clients[client];
     release_range(sv, c.start, c.count);
     sv->used -= c.count;
     c.count = 0;

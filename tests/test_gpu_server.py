@@ -127,3 +127,52 @@ def test_device_submission_matches_host(mods):
     assert np.array_equal(sv.observe(b), root)
     assert sv.stats()["requests"] == 2 * steps
     sv.close_server()
+
+
+def test_open_close_while_stepping(mods):
+    """Clients opened, stepped and closed by several threads while another
+    client steps: every request completes, no range is handed out twice while
+    in use, and the worlds of the client that stays open evolve exactly as the
+    same worlds in a sim of their own."""
+    native, server = mods
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(600 * (per + 96))
+    n, steps = 50, 4
+    keep = sv.open(n, seed0=7, difficulty=0.5)
+    acts = action_stream(n, steps, seed=3).astype(np.float32)
+    errors, ranges = [], []
+    lock = threading.Lock()
+
+    def churn(i):
+        try:
+            m = 20 + i
+            t = action_stream(m, 1, seed=i)[0].astype(np.float32)
+            for r in range(5):
+                c = sv.open(m, seed0=100 * i + r)
+                with lock:
+                    ranges.append(sv.range(c))
+                for _ in range(2):
+                    sv.wait(sv.submit(c, t))
+                sv.close(c)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=churn, args=(i,)) for i in range(6)]
+    for x in th:
+        x.start()
+    for k in range(steps):
+        sv.wait(sv.submit(keep, acts[k]))
+    for x in th:
+        x.join()
+    assert not errors
+    s0, c0 = sv.range(keep)
+    assert all(s + c <= s0 or s >= s0 + c0 for s, c in ranges)  # never inside the open client's range
+    assert sv.used == n
+    ref = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    ref.reset(ids, (ids + 7).astype(np.uint32), np.full(n, 0.5))
+    for k in range(steps):
+        ref.step(acts[k].astype(np.float64))
+    assert np.array_equal(sv.observe(keep), ref.observe()[0])
+    assert sv.stats()["requests"] == steps + 6 * 5 * 2
+    sv.close_server()
