@@ -24,13 +24,14 @@ namespace {
 constexpr int RING = 4;   /* pinned request tables in flight */
 constexpr int GRAPHS = 4; /* captured graphs kept per handle */
 
+/* a captured launch sequence: the step kernels read every range's targets
+ * through the table's row pointers, so no buffer address is part of it */
 struct LaunchKey {
-    const float *targets;
     int task, block, tile, grid, persistent, leg;
     bool operator==(const LaunchKey &o) const
     {
-        return targets == o.targets && task == o.task && block == o.block && tile == o.tile && grid == o.grid
-               && persistent == o.persistent && leg == o.leg;
+        return task == o.task && block == o.block && tile == o.tile && grid == o.grid && persistent == o.persistent
+               && leg == o.leg;
     }
 };
 
@@ -184,7 +185,7 @@ static hipError_t graph_for(arc_sim *s, BatchState *b, const LaunchKey &k, hipGr
         hipError_t r = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
         if (r != hipSuccess)
             return r;
-        hipError_t r1 = enqueue(s, b, k.targets, k.task, k.grid);
+        hipError_t r1 = enqueue(s, b, nullptr, k.task, k.grid);
         hipError_t r2 = hipStreamEndCapture(s->stream, &e->graph);
         if (r1 != hipSuccess)
             return r1;
@@ -259,13 +260,20 @@ int arcx_get_launch(const arc_sim *s, int *out5)
     return 0;
 }
 
-int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
-                    int graph)
+/* arcx_step_batch and arcx_step_batch_rows: request r's targets start at
+ * rows[r] (rows given), else at targets_dev + starts[r] x 12 */
+static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev,
+                      const float *const *rows, int task, int graph)
 {
-    if (n_req < 0 || n_req > MAX_BATCH || (n_req && (!starts || !counts)) || !targets_dev) {
+    if (n_req < 0 || n_req > MAX_BATCH || (n_req && (!starts || !counts)) || (!targets_dev && !rows)) {
         set_error(s, -1, "arcx_step_batch: request list", hipSuccess);
         return -1;
     }
+    for (int r = 0; rows && r < n_req; ++r)
+        if (counts[r] > 0 && !rows[r]) {
+            set_error(s, -1, "arcx_step_batch_rows: a request without targets", hipSuccess);
+            return -1;
+        }
     /* ranges inside [0, N) and disjoint: overlapping ranges would race */
     std::vector<std::pair<int, int>> iv;
     iv.reserve(n_req);
@@ -305,6 +313,7 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
         h->r[m].count = counts[r];
         h->r[m].prefix = total;
         h->r[m].pad = 0;
+        h->r[m].targets = rows ? rows[r] : targets_dev + (size_t)starts[r] * NJ;
         total += counts[r];
         ++m;
     }
@@ -312,7 +321,8 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
     h->total = total;
     if (m == 0) {
         /* an empty table still steps nothing: the kernels read total = 0 */
-        h->r[0].start = h->r[0].count = h->r[0].prefix = 0;
+        h->r[0].start = h->r[0].count = h->r[0].prefix = h->r[0].pad = 0;
+        h->r[0].targets = nullptr;
     }
     const size_t bytes = offsetof(StepBatch, r) + sizeof(Range) * (size_t)std::max(m, 1);
     /* the device table is rewritten only when the batch differs from the last
@@ -327,15 +337,26 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
     if (!graph) {
         if (total == 0 && !task)
             return 0;
-        HIPCHK_RET(s, enqueue(s, b, targets_dev, task, grid_for(s, std::max(total, 1))), "arcx_step_batch launch", -1);
+        HIPCHK_RET(s, enqueue(s, b, nullptr, task, grid_for(s, std::max(total, 1))), "arcx_step_batch launch", -1);
         return 0;
     }
-    LaunchKey k{ targets_dev, task ? 1 : 0, s->block, tile_of(s), grid_for(s, std::max(s->n, 1)), s->persistent,
-                 is_leg(s) };
+    LaunchKey k{ task ? 1 : 0, s->block, tile_of(s), grid_for(s, std::max(s->n, 1)), s->persistent, is_leg(s) };
     hipGraphExec_t exec = nullptr;
     HIPCHK_RET(s, graph_for(s, b, k, &exec), "arcx_step_batch capture", -1);
     HIPCHK_RET(s, hipGraphLaunch(exec, s->stream), "arcx_step_batch graph", -1);
     return 0;
+}
+
+int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
+                    int graph)
+{
+    return step_batch(s, starts, counts, n_req, targets_dev, nullptr, task, graph);
+}
+
+int arcx_step_batch_rows(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
+                         int task, int graph)
+{
+    return step_batch(s, starts, counts, n_req, nullptr, rows, task, graph);
 }
 
 int arcx_graph_captures(const arc_sim *s) { return s->batch ? s->batch->captures : 0; }

@@ -91,9 +91,12 @@ struct TaskState {
 };
 
 /* Slices of worlds a batched launch steps: worlds [start, start+count)
- * for each range; prefix = sum of the counts before it. */
+ * for each range; prefix = sum of the counts before it; the range's targets
+ * are count x 12 floats from `targets` (its request's own device buffer, or
+ * the rows of a shared buffer). */
 struct Range {
     int start, count, prefix, pad;
+    const float *targets;
 };
 /* requests per batched launch */
 constexpr int MAX_BATCH = 256;
@@ -121,6 +124,22 @@ __device__ __forceinline__ int batch_world(const StepBatch *__restrict__ b, int 
             hi = mid - 1;
     }
     return b->r[lo].start + (t - b->r[lo].prefix);
+}
+/* the same, with the first of the world's 12 targets */
+__device__ __forceinline__ int batch_world(const StepBatch *__restrict__ b, int n_ranges, int t,
+                                           const float *__restrict__ &row)
+{
+    int lo = 0, hi = n_ranges - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (b->r[mid].prefix <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int off = t - b->r[lo].prefix;
+    row = b->r[lo].targets + (size_t)off * NJ;
+    return b->r[lo].start + off;
 }
 
 /* Kernel launchers (kernels.hip).  All are asynchronous on `stream`. */

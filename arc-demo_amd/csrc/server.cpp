@@ -4,10 +4,10 @@
  * One arc_sim holds the worlds of every admitted client, sized once by the
  * HBM budget.  Clients get contiguous world ranges (first fit).  Requests
  * queue per client; a dispatcher thread forms each batch from the oldest
- * pending request of every client, copies their targets into the pinned
- * staging rows of their worlds and uploads those rows (or, for targets a
- * client already holds in device memory, copies them device to device into
- * the same rows), and replays the captured
+ * pending request of every client, copies host targets into the pinned
+ * staging rows of their worlds and uploads those rows (targets a client
+ * already holds in device memory are read where they lie: the batch table
+ * carries each request's row address), and replays the captured
  * batch graph (step + fall rules + auto-reset of the batch's worlds) without
  * waiting for it; a completer thread waits for each batch's event in launch
  * order and stamps its requests done.  Two batches may be in flight, each
@@ -116,6 +116,7 @@ static void release_range(arc_server *sv, int start, int count)
 static void dispatch_loop(arc_server *sv)
 {
     std::vector<int> starts, counts;
+    std::vector<const float *> rows;
     for (;;) {
         Flight f;
         {
@@ -155,27 +156,32 @@ static void dispatch_loop(arc_server *sv)
         }
         starts.clear();
         counts.clear();
+        rows.clear();
         {
             std::lock_guard<std::mutex> sl(sv->sim_mu);
             arc_sim *s = sv->sim;
             float *h = sv->h_targets[f.buf], *d = sv->d_targets[f.buf];
+            bool uploads = false;
             for (auto &r : f.reqs) {
                 const size_t off = (size_t)r.start * NJ, bytes = (size_t)r.count * NJ * sizeof(float);
                 if (r.dev) {
-                    f.ok = f.ok && hipMemcpyAsync(d + off, r.dev, bytes, hipMemcpyDeviceToDevice, sv->copy)
-                                       == hipSuccess;
+                    rows.push_back(r.dev); /* read where the client holds it */
                 } else {
                     memcpy(h + off, r.targets.data(), bytes);
                     f.ok = f.ok && hipMemcpyAsync(d + off, h + off, bytes, hipMemcpyHostToDevice, sv->copy)
                                        == hipSuccess;
+                    rows.push_back(d + off);
+                    uploads = true;
                 }
                 starts.push_back(r.start);
                 counts.push_back(r.count);
                 f.worlds += r.count;
             }
-            f.ok = f.ok && hipEventRecord(sv->copied[f.buf], sv->copy) == hipSuccess
-                   && hipStreamWaitEvent(s->stream, sv->copied[f.buf], 0) == hipSuccess;
-            f.ok = f.ok && arcx_step_batch(s, starts.data(), counts.data(), (int)f.reqs.size(), d, 1, 1) == 0;
+            if (uploads)
+                f.ok = f.ok && hipEventRecord(sv->copied[f.buf], sv->copy) == hipSuccess
+                       && hipStreamWaitEvent(s->stream, sv->copied[f.buf], 0) == hipSuccess;
+            f.ok = f.ok && arcx_step_batch_rows(s, starts.data(), counts.data(), (int)f.reqs.size(), rows.data(), 1, 1)
+                               == 0;
             f.ok = f.ok && hipEventRecord(sv->ev[f.buf], s->stream) == hipSuccess;
         }
         {

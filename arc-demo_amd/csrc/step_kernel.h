@@ -1,13 +1,15 @@
 /*
- * step_kernel.h -- the lane-form step kernel (one world per lane, all
- * substeps of a control step in one launch) and its launcher, instantiated
+ * step_kernel.h -- the register-resident step kernels (one world per lane,
+ * four or two lanes per world, four lanes with shared root solves; all
+ * substeps of a control step in one launch) and their launcher, instantiated
  * once per workgroup size by step_b64.hip / step_b128.hip / step_b256.hip so
  * the three compile in parallel.
  *
  * Work distribution, per launch:
- *   plain      a grid-stride loop over worlds 0..total-1;
+ *   plain      a grid-stride loop over worlds 0..total-1, targets world-major;
  *   batched    the same over a StepBatch read from device memory at run time
- *              (uneven world ranges of several requests; one captured graph
+ *              (uneven world ranges of several requests, each range's targets
+ *              read from the buffer its row pointer names; one captured graph
  *              serves any batch the host writes into that table);
  *   persistent a grid sized to the resident wavefronts, each wavefront taking
  *              the next 64 worlds from an atomic counter until none are left.
@@ -176,14 +178,14 @@ __device__ __forceinline__ size_t opaque_index(size_t i)
 
 /* one control step of world i by the calling lane */
 template <int BLOCK, int TILE>
-__device__ __forceinline__ void step_world(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+__device__ __forceinline__ void step_world(const DevWorlds &d, const Model &M, const float *__restrict__ row,
                                            LdsPoses<BLOCK> &P0, int8_t *s_tile, int i)
 {
     const size_t S = d.stride;
     LaneWorld W;
-    /* targets clamped to the limits, csrc/arc_sim.c:132-134 */
+    /* targets (row: the world's 12) clamped to the limits, csrc/arc_sim.c:132-134 */
     static_for<0, NJ>([&](auto j) {
-        float tg = targets[(size_t)i * NJ + j];
+        float tg = row[j];
         W.target[j] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
         d.target[j * S + i] = W.target[j];
     });
@@ -241,8 +243,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
     int base = work ? next(0) : (int)blockIdx.x * BLOCK + (threadIdx.x & ~63);
     while (base < total) {
         const int t = base + lane;
-        if (t < total)
-            step_world<BLOCK, TILE>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, t) : t);
+        if (t < total) {
+            const float *row = nullptr;
+            const int i = batch ? batch_world(batch, n_ranges, t, row) : t;
+            if (!batch)
+                row = targets + (size_t)t * NJ;
+            step_world<BLOCK, TILE>(d, M, row, P0, s_tile, i);
+        }
         base = next(base);
     }
 }
@@ -317,7 +324,7 @@ __device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, c
 
 /* one control step of world i by lane q of its group */
 template <int BLOCK, int TILE, int K>
-__device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+__device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ row,
                                                LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q)
 {
     constexpr int P = LEGS / K;
@@ -325,7 +332,7 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
     LegWorld<K> W;
     for (int m = 0; m < 3 * K; ++m) {
         const int j = 3 * q * K + m;
-        float tg = targets[(size_t)i * NJ + j];
+        float tg = row[j];
         W.target[m] = clampf_(tg, M.joint[j].lo, M.joint[j].hi);
         d.target[j * S + i] = W.target[m];
     }
@@ -386,8 +393,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
         const int t = base + lane;
         if (t < items) {
             const int w = t / P;
-            step_world_leg<BLOCK, TILE, K>(d, M, targets, P0, s_tile, batch ? batch_world(batch, n_ranges, w) : w,
-                                           t % P);
+            const float *row = nullptr;
+            const int i = batch ? batch_world(batch, n_ranges, w, row) : w;
+            if (!batch)
+                row = targets + (size_t)w * NJ;
+            step_world_leg<BLOCK, TILE, K>(d, M, row, P0, s_tile, i, t % P);
         }
         base = next(base);
     }
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
  * world's state with their anchors and contact lists in the dummy area and
  * store nothing. */
 template <int BLOCK>
-__device__ __forceinline__ void step_world_shared(const DevWorlds &d, const Model &M, const float *__restrict__ targets,
+__device__ __forceinline__ void step_world_shared(const DevWorlds &d, const Model &M, const float *__restrict__ row,
                                                   LdsPoses<BLOCK> &P0, int8_t *s_tile, RootsShared<BLOCK / 4> &roots,
                                                   int i, int q, bool valid)
 {
@@ -409,7 +419,7 @@ __device__ __forceinline__ void step_world_shared(const DevWorlds &d, const Mode
     LegWorld<1> W;
     for (int m = 0; m < 3; ++m) {
         const int j = 3 * q + m;
-        W.target[m] = clampf_(targets[(size_t)i * NJ + j], M.joint[j].lo, M.joint[j].hi);
+        W.target[m] = clampf_(row[j], M.joint[j].lo, M.joint[j].hi);
         if (valid)
             d.target[j * S + i] = W.target[m];
     }
@@ -464,8 +474,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
         const int t = base + (int)threadIdx.x;
         const bool valid = t < items;
         const int w = (valid ? t : items - 1) >> 2;
-        step_world_shared<BLOCK>(d, M, targets, P0, s_tile, roots, batch ? batch_world(batch, n_ranges, w) : w,
-                                 (int)threadIdx.x & 3, valid);
+        const float *row = nullptr;
+        const int i = batch ? batch_world(batch, n_ranges, w, row) : w;
+        if (!batch)
+            row = targets + (size_t)w * NJ;
+        step_world_shared<BLOCK>(d, M, row, P0, s_tile, roots, i, (int)threadIdx.x & 3, valid);
     }
 }
 

@@ -116,12 +116,20 @@ int arcx_get_launch(const arc_sim *s, int *out5);
  * world-major (rows of worlds not in the batch are not read).  Ranges must be
  * inside [0, N) and disjoint (checked; -1 otherwise).  task = 1 adds the fall
  * rules and auto-reset of the stepped worlds (as arcx_step_task).  graph = 1
- * replays a captured hipGraph of the launch sequence: one graph per
- * (targets_dev, task, launch configuration), the last four kept, while the
- * request table is read from device memory at every replay.  Asynchronous;
- * arcx_sync waits. */
+ * replays a captured hipGraph of the launch sequence: one graph per (task,
+ * launch configuration), the last four kept, while the request table (ranges
+ * and the address of each range's targets) is read from device memory at
+ * every replay.  Asynchronous; arcx_sync waits. */
 int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
                     int graph);
+/* As arcx_step_batch, with each request's targets in memory of its own:
+ * request r's worlds read rows[r] + 12 k, k < counts[r] (float32 device
+ * memory, e.g. the output tensor of a client's policy, read where it lies:
+ * no copy).  The memory is read when the launch runs and must hold the
+ * targets until the handle's stream has passed the step.  Same graphs as
+ * arcx_step_batch; -1 if a non-empty request has no rows. */
+int arcx_step_batch_rows(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
+                         int task, int graph);
 /* graphs captured so far by arcx_step_batch */
 int arcx_graph_captures(const arc_sim *s);
 

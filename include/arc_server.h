@@ -10,8 +10,8 @@
  *   - batching: step requests are queued as they arrive; a dispatcher thread
  *     takes the oldest pending request of every client (waiting up to a
  *     batching window for the clients that have none yet), uploads their
- *     targets (or copies them device to device when the client submitted
- *     device memory) and steps all their worlds in one batched launch
+ *     host targets (device targets are read where the client holds them)
+ *     and steps all their worlds in one batched launch
  *     (arcx_step_batch with the fall rules and auto-reset, replayed from a
  *     hipGraph);
  *   - latency: each request is timed from its submission to the end of the batch
@@ -54,8 +54,8 @@ long long arcs_submit(arc_server *sv, int client, const float *targets);
 /* Queues one control step of a client whose targets are already in device
  * memory: targets_dev = n_worlds x 12 float32 on the server's device, written
  * completely before the call (the caller synchronises its producer) and left
- * unchanged until arcs_wait on the ticket returns.  The dispatcher copies them
- * device to device into the batch's targets rows; nothing crosses PCIe.
+ * unchanged until arcs_wait on the ticket returns.  The step kernels read them
+ * where they lie (no copy; nothing crosses PCIe).
  * Returns a ticket > 0, or -1.  Requests of either kind may be mixed. */
 long long arcs_submit_device(arc_server *sv, int client, const float *targets_dev);
 /* Blocks until the ticket's step is done; returns its latency in

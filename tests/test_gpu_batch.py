@@ -233,3 +233,35 @@ def test_leg_form_batches_and_graphs(native, form, persistent):
         b.sync()
         assert np.array_equal(_bits(a, n), _bits(b, n)), f"batch {k}"
         assert np.array_equal(a.task_causes(), b.task_causes())
+
+
+def test_batch_rows_match_one_buffer(native):
+    """Each request's targets in memory of its own (arcx_step_batch_rows, read
+    where they lie) step the worlds exactly as the same targets in one
+    world-major buffer: leg form replayed from one graph whose table changes
+    its row addresses every step, and the lane and shared-root forms launched
+    eagerly; with the fall rules and auto-reset on."""
+    n, steps = 300, 3
+    ts = action_stream(n, steps, seed=11).astype(np.float32)
+    starts, counts = [0, 40, 170, 299], [25, 100, 90, 1]
+    dev = native.DeviceBuffer.from_array(ts)
+    per = n * 12 * 4
+    bufs = [[native.DeviceBuffer.from_array(np.ascontiguousarray(ts[k, s0:s0 + c]))
+             for s0, c in zip(starts, counts)] for k in range(steps)]
+    ref, rows_graph, rows_lane, rows_shared = (_sim(native, n) for _ in range(4))
+    rows_lane.set_form("lane")
+    rows_shared.set_form("shared")
+    for k in range(steps):
+        ref.step_batch(starts, counts, dev.ptr + k * per, task=True, graph=True)
+        rp = [b.ptr for b in bufs[k]]
+        rows_graph.step_batch_rows(starts, counts, rp, task=True, graph=True)
+        rows_lane.step_batch_rows(starts, counts, rp, task=True)
+        rows_shared.step_batch_rows(starts, counts, rp, task=True)
+        for s in (ref, rows_graph, rows_lane, rows_shared):
+            s.sync()
+        want = _bits(ref, n)
+        for s, name in ((rows_graph, "graph"), (rows_lane, "lane"), (rows_shared, "shared")):
+            assert np.array_equal(_bits(s, n), want), f"{name}, step {k}"
+    assert rows_graph.graph_captures() == 1
+    with pytest.raises(native.SimError):
+        rows_graph.step_batch_rows([0], [5], [0])  # a request without targets
