@@ -325,10 +325,17 @@ void arc_sim_observe(const arc_sim *cs, double *root, double *rot, double *joint
     HIPCHK(s, launch_observe(s->dev, s->d_model, root ? d_root : nullptr, rot ? d_rot : nullptr,
                              joints ? d_j : nullptr, contacts ? d_c : nullptr, flags ? d_f : nullptr, s->stream),
            "arc_sim_observe launch");
-    HIPCHK(s, hipMemcpyAsync(s->h_pin, s->d_io, n * per * sizeof(double), hipMemcpyDeviceToHost, s->stream),
-           "arc_sim_observe download");
-    HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_observe");
+    /* only the requested outputs cross PCIe (each is one contiguous section) */
     double *h = (double *)s->h_pin;
+    const struct { const void *want; size_t off, bytes; } part[5] = {
+        { root, 0, n * 13 * sizeof(double) },          { rot, n * 13, n * 9 * sizeof(double) },
+        { joints, n * 22, n * 36 * sizeof(double) },   { contacts, n * 58, n * 6 * sizeof(double) },
+        { flags, n * 64, n * sizeof(int) } };
+    for (const auto &p : part)
+        if (p.want)
+            HIPCHK(s, hipMemcpyAsync(h + p.off, s->d_io + p.off, p.bytes, hipMemcpyDeviceToHost, s->stream),
+                   "arc_sim_observe download");
+    HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_observe");
     if (root) memcpy(root, h, n * 13 * sizeof(double));
     if (rot) memcpy(rot, h + n * 13, n * 9 * sizeof(double));
     if (joints) memcpy(joints, h + n * 22, n * 36 * sizeof(double));

@@ -1,5 +1,8 @@
 """Edge cases of the step path on the GPU, through the C ABI.
 
+- arc_sim_observe with any subset of its outputs (the others NULL, as the
+  reference allows): each requested output equals the full read-out's, and
+  the buffers not requested are left untouched;
 - a sim of no worlds: create, step, observe, heights and a batch of no
   requests do nothing and report no error (the reference's loops are empty);
 - world counts that fill no whole wavefront or quad group (1, 3, 17, 100):
@@ -37,6 +40,27 @@ def native():
 
 def _states(sim, n):
     return np.stack([sim.state(i) for i in range(n)])
+
+
+def test_observe_output_subsets(native):
+    n = 37
+    g = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), P.difficulties(4, n))
+    g.step(action_stream(n, 1)[0])
+    root, joints, contacts, flags = (x.copy() for x in g.observe())
+    rot = g.rot.copy()
+    full = [root, rot, joints, contacts, flags]
+    for k in range(5):
+        bufs = [np.full_like(x, 7.0) if x.dtype != np.int32 else np.full_like(x, 7) for x in full]
+        args = [b.ctypes.data if j == k else None for j, b in enumerate(bufs)]
+        g._lib.arc_sim_observe(g.ptr, *args)
+        for j, b in enumerate(bufs):
+            if j == k:
+                assert np.array_equal(b, full[j]), f"output {j}"
+            else:
+                assert np.all(b == 7), f"output {j} written though not requested"
+    g.close()
 
 
 def test_no_worlds(native):
