@@ -76,6 +76,7 @@ def actions_to_targets(defaults: np.ndarray, u: np.ndarray) -> np.ndarray:
 
 
 MAX_SETS = 512
+TUNE_STEPS = 8   # timed steps per candidate in the autotuner (fewer let run-to-run noise pick a slower shape)
 
 
 def target_pool(defaults: np.ndarray, n: int, sets: int, seed: int = 0) -> np.ndarray:
@@ -115,7 +116,7 @@ def warm_and_tune(s, n, t, warm, sets):
     for k in range(warm):
         s.step_batch([0], [n], t.ptr + 4 * per * (k % sets), task=True, graph=True)
     s.sync()
-    cfg, _ = s.autotune(3)
+    cfg, _ = s.autotune(TUNE_STEPS)
     return cfg
 
 
@@ -293,7 +294,7 @@ def served_pass(steps: int, warm: int) -> dict:
 
     out = {"clients": CLIENTS, "worlds_per_client": n, "targets": "device-resident (arcs_submit_device)"}
     run(0, warm, 1, True)
-    out["launch"], _ = sv.autotune(3)
+    out["launch"], _ = sv.autotune(TUNE_STEPS)
     k = warm
     for depth, device, name in ((1, True, "closed_loop"), (2, True, "pipelined"), (2, False, "pipelined_host_pcie")):
         run(k, warm, depth, device)
@@ -335,7 +336,7 @@ def serving_pass(seconds: float) -> dict:
     for _ in range(30):  # settle the worlds, then tune the launch on them
         for x in [sv.submit(c, acts(c)) for c in cids]:
             sv.wait(x)
-    launch, _ = sv.autotune(3)
+    launch, _ = sv.autotune(TUNE_STEPS)
     # saturation: every client keeps one request in flight
     stop = time.perf_counter() + min(1.0, seconds / 3)
     for c in cids:
@@ -408,7 +409,7 @@ def rollout_task(steps: int, rollouts: int, n: int, mode: str, ordering: str) ->
     t = devtask.DeviceTask(n, cfg, seed=0, rng=mode)
     t.set_levels(unit24(rng(1, n)))
     t.reset()
-    cfgl, _ = t.sim.autotune(3)
+    cfgl, _ = t.sim.autotune(TUNE_STEPS)
     dev = torch.device("cuda")
     torch.manual_seed(0)
     policy = torch.nn.Sequential(torch.nn.Linear(t.obs_size, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
