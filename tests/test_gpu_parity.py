@@ -176,14 +176,17 @@ def test_wave_form_matches_lane_form(native, difficulty):
     assert np.all(np.abs(a - c) / (np.abs(c) + 1.0) <= 1e-4)
 
 
+@pytest.mark.parametrize("form", ["lane", "leg", "pair", "shared"])
 @pytest.mark.parametrize("grid,cell", [(8, 0.4), (16, 0.4), (20, 0.25)])
-def test_small_grids(native, grid, cell):
-    """Grids where the robot's legs reach the border and beyond: on the 8-column
-    grid no LDS tile fits and every column is read from HBM; on the others the
-    tile is clamped to the padded grid and the legs read columns outside it."""
+def test_small_grids(native, grid, cell, form):
+    """Grids where the robot's legs reach the border and beyond, in every
+    register-resident form: on the 8-column grid no LDS tile fits and every
+    column is read from HBM; on the others the tile is clamped to the padded
+    grid and the legs read columns outside it."""
     n = 64
     diff = P.difficulties(1, n)
     g = native.Sim(n, grid=grid, cell=cell)
+    g.set_form(form)
     o = P.Lib("o32").sim(n, grid=grid, cell=cell)
     ids = np.arange(n, dtype=np.int32)
     g.reset(ids, ids.astype(np.uint32), diff)
@@ -195,11 +198,12 @@ def test_small_grids(native, grid, cell):
     o.step(t)
     a, b = _states(g, n), _states(o, n)
     err = np.abs(a - b) / (np.abs(b) + 1.0)
-    print(f"grid {grid} cell {cell}: max rel err {float(err.max()):.3g}, bit-exact worlds {same(a, b).all(1).mean():.3f}")
+    print(f"{form}, grid {grid} cell {cell}: max rel err {float(err.max()):.3g}, "
+          f"bit-exact worlds {same(a, b).all(1).mean():.3f}")
     assert float(err.max()) <= REL
 
 
-@pytest.mark.parametrize("form", ["lane", "leg"])
+@pytest.mark.parametrize("form", ["lane", "leg", "pair", "shared"])
 @pytest.mark.parametrize("dt,substeps,grid,cell", [(1.0 / 60.0, 10, 64, 0.4), (1.0 / 20.0, 32, 48, 0.5)])
 def test_other_timesteps_and_grids(native, form, dt, substeps, grid, cell):
     """Step length, substep count, grid and cell other than the defaults: the
