@@ -16,13 +16,15 @@ from helpers import action_stream
 pytestmark = pytest.mark.gpu
 
 
-def test_falls_and_autoreset_match_oracle():
+@pytest.mark.parametrize("form", ["lane", "leg", "pair", "shared"])
+def test_falls_and_autoreset_match_oracle(form):
     from arc import native
 
     n, steps = 96, 6
     beached, flip, void = 2, 0.97, -2.0
     diff = P.difficulties(1, n)
     g = native.Sim(n)
+    g.set_form(form)
     o = P.Lib("o32").sim(n)
     ids = np.arange(n, dtype=np.int32)
     g.reset(ids, ids.astype(np.uint32), diff)

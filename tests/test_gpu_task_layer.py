@@ -40,8 +40,8 @@ def _same_state(t, rng):
             and (not b["has_uint32"] or a["uinteger"] == b["uinteger"]))
 
 
-@pytest.mark.parametrize("form,mode", [("lane", "world"), ("leg", "world"), ("leg", "numpy"), ("lane", "numpy"),
-                                       ("shared", "numpy")])
+@pytest.mark.parametrize("form,mode", [("lane", "world"), ("leg", "world"), ("pair", "world"), ("shared", "world"),
+                                       ("leg", "numpy"), ("lane", "numpy"), ("shared", "numpy")])
 def test_device_task_matches_oracle(form, mode):
     from arc import devtask
     n, steps = 96, 40
