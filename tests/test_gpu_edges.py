@@ -1,5 +1,8 @@
 """Edge cases of the step path on the GPU, through the C ABI.
 
+- non-finite targets: NaN passes the reference's clamp and makes its world
+  non-finite and unstable (frozen from then on), +-inf clamp to the joint
+  limits; every form against the oracle (NaN == NaN), flags exact;
 - arc_sim_observe with any subset of its outputs (the others NULL, as the
   reference allows): each requested output equals the full read-out's, and
   the buffers not requested are left untouched;
@@ -40,6 +43,34 @@ def native():
 
 def _states(sim, n):
     return np.stack([sim.state(i) for i in range(n)])
+
+
+def test_non_finite_targets(native):
+    n, steps = 16, 3
+    diff = P.difficulties(5, n)
+    ids = np.arange(n, dtype=np.int32)
+    ts = action_stream(n, steps, seed=8)
+    ts[0, 2, 4] = np.nan
+    ts[0, 5, :] = np.inf
+    ts[0, 7, 1] = -np.inf
+    ts[1, 9, 0] = np.nan
+    o = P.Lib("o32").sim(n)
+    o.reset(ids, ids.astype(np.uint32), diff)
+    want = []
+    for k in range(steps):
+        o.step(ts[k])
+        want.append((_states(o, n), o.observe()[4].copy()))
+    assert want[-1][1][2] and want[-1][1][9] and not want[-1][1][5]  # NaN worlds unstable, inf world clamped
+    for form in ("lane", "leg", "pair", "shared"):
+        g = native.Sim(n)
+        g.set_form(form)
+        g.reset(ids, ids.astype(np.uint32), diff)
+        for k in range(steps):
+            g.step(ts[k])
+            a, flags = _states(g, n), g.observe()[3]
+            assert same(a, want[k][0]).all(), f"{form}, step {k + 1}"
+            assert np.array_equal(flags, want[k][1]), f"{form}, step {k + 1}"
+        g.close()
 
 
 def test_observe_output_subsets(native):
