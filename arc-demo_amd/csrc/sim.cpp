@@ -253,16 +253,25 @@ void arc_sim_reset(arc_sim *s, const int *ids, int n_ids, const unsigned int *se
         return;
     /* last entry per world wins (the reference resets them in list order) */
     std::vector<int> last(s->n, -1);
-    int bad = 0;
+    int bad = 0, steep = 0;
     for (int k = 0; k < n_ids; ++k) {
         if (ids[k] < 0 || ids[k] >= s->n) {
             ++bad;
+            continue;
+        }
+        /* the device terrain holds each column as an int8 count of 0.03 m
+         * steps, |F| <= 21 d + 1 before the re-basing: difficulties up to
+         * MAX_DIFFICULTY fit (NaN and d <= 0 give the flat terrain) */
+        if (difficulty[k] > MAX_DIFFICULTY) {
+            ++steep;
             continue;
         }
         last[ids[k]] = k;
     }
     if (bad)
         set_error(s, -1, "arc_sim_reset: world id out of range (skipped)", hipSuccess);
+    if (steep)
+        set_error(s, -1, "arc_sim_reset: difficulty above 6, beyond the device terrain's range (skipped)", hipSuccess);
     std::vector<int> kid;
     kid.reserve(n_ids);
     for (int k = 0; k < n_ids; ++k)

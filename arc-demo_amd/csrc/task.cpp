@@ -395,6 +395,11 @@ int arct_numpy_state(const arc_task *ct, unsigned long long *out6)
 int arct_set_levels(arc_task *t, const double *levels)
 {
     arc_sim *s = t->sim;
+    for (int i = 0; i < t->n; ++i)
+        if (levels[i] > MAX_DIFFICULTY) {
+            set_error(s, -1, "arct_set_levels: level above 6, beyond the device terrain's range", hipSuccess);
+            return -1;
+        }
     HIPCHK_RET(s, hipMemcpyAsync(t->td.level, levels, (size_t)t->n * sizeof(double), hipMemcpyHostToDevice, s->stream),
                "arct_set_levels", -1);
     HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arct_set_levels", -1);

@@ -504,6 +504,10 @@ struct SphereK {
  * the grid read as the world's `outside` height. */
 constexpr int TPAD = 2;
 constexpr int Q_OUT = -128;
+/* largest terrain difficulty whose columns fit the int8 encoding: a column is
+ * F 0.03 m steps with |F| <= 21 d + 1 (hills 0.45 d, bumps 0.12 d, jitter
+ * 0.06 d, csrc/arc_terrain.c:54-71), so d <= 6 keeps |F| <= 127 */
+constexpr double MAX_DIFFICULTY = 6.0;
 struct TerrainRef {
     const int8_t *q;  /* column (0, 0); row pitch `pitch` */
     int pitch;        /* nx + 2 * TPAD */

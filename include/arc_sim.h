@@ -14,6 +14,9 @@
  *     on stderr); the reference returns NULL only when out of memory;
  *   - out-of-range world ids in arc_sim_reset / arc_sim_terrain_at are
  *     skipped and reported (the reference does not check them);
+ *   - arc_sim_reset skips and reports entries whose difficulty exceeds 6:
+ *     the device stores terrain columns as int8 counts of 0.03 m, which
+ *     holds every column up to that difficulty (the task uses [0, 1]);
  *   - arc_sim_world returns a snapshot copied from the device on each call
  *     (valid until the next call on the handle), not a live pointer, and it
  *     carries no render-only visuals;

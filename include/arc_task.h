@@ -103,7 +103,8 @@ int arct_seed_numpy(arc_task *t, const unsigned int *words, int n_words);
 int arct_numpy_state(const arc_task *t, unsigned long long *out6);
 
 /* curriculum levels of every world (N doubles, host): the difficulty of each
- * world's next episode (with curriculum off, its fixed difficulty) */
+ * world's next episode (with curriculum off, its fixed difficulty); each at
+ * most 6, the device terrain's range (-1 and nothing set otherwise) */
 int arct_set_levels(arc_task *t, const double *levels);
 
 /* host read-backs: checkpoints (N x 3: x, y, terrain z), curriculum levels

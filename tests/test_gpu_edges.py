@@ -14,7 +14,8 @@
   oracle after one step, and bit-identical to each other;
 - reset inputs: an empty list (nothing changes), repeated ids (the last entry
   wins, as resetting in list order does), ids out of range (skipped, the
-  others reset, the error reported);
+  others reset, the error reported), difficulties beyond the int8 terrain's
+  range (skipped and reported; 6 itself exact against the oracle);
 - the largest batch: 256 requests of uneven sizes, some of one world, step
   exactly what one step of all worlds does;
 - the full size, 65,536 worlds: the forms agree bit for bit after a few steps
@@ -156,8 +157,21 @@ def test_reset_edge_inputs(native):
         g.reset(np.array([-1, 3, n], np.int32), np.array([1, 55, 2], np.uint32), np.array([0.5, 0.5, 0.5]))
     h.reset(np.array([3], np.int32), np.array([55], np.uint32), np.array([0.5]))
     assert same(_states(g, n), _states(h, n)).all()
+    # difficulty above 6 does not fit the device terrain: that entry is skipped and reported
+    with pytest.raises(native.SimError):
+        g.reset(np.array([1, 4], np.int32), np.array([9, 10], np.uint32), np.array([6.5, 0.7]))
+    h.reset(np.array([4], np.int32), np.array([10], np.uint32), np.array([0.7]))
+    assert same(_states(g, n), _states(h, n)).all()
     g.close()
     h.close()
+    # the steepest accepted terrain, against the oracle
+    o = P.Lib("o32").sim(4)
+    g = native.Sim(4)
+    for s in (o, g):
+        s.reset(np.arange(4, dtype=np.int32), np.array([3, 77, 1234, 99999], np.uint32), np.full(4, 6.0))
+    for i in range(4):
+        assert np.array_equal(g.terrain(i), o.terrain(i)), f"terrain of world {i} at difficulty 6"
+    g.close()
 
 
 def test_largest_batch_equals_full_step(native):

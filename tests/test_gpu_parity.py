@@ -59,8 +59,11 @@ def test_template_matches_oracle(native):
     assert g.stand_height == misc[0] and g.total_mass == misc[1]
 
 
-@pytest.mark.parametrize("difficulty", [0.0, 0.5, 1.0, "mixed"])
+@pytest.mark.parametrize("difficulty", [-0.5, 0.0, 1e-7, 0.5, 1.0, 1.7, "mixed"])
 def test_reset_bit_exact(native, difficulty):
+    """Terrain exact and pose within the bar after a reset: at and below zero
+    difficulty the terrain is flat and draws nothing; a tiny positive one draws
+    the full stream; above one the amplitudes only grow."""
     n = 48
     diff = P.difficulties(1, n) if difficulty == "mixed" else np.full(n, difficulty)
     g, o = _pair(native, n, diff)
@@ -71,6 +74,27 @@ def test_reset_bit_exact(native, difficulty):
     close = np.abs(a - b) <= REL * (np.abs(b) + 1.0)
     assert close.all()
     assert same(a, b).all(1).mean() > 0.9
+
+
+def test_reset_seed_extremes(native):
+    """Seeds at the ends of the 32-bit range (the hash seeding and the jump
+    tables treat every seed alike): terrain and state after reset, then one
+    step, against the oracle."""
+    seeds = np.array([0, 1, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 2, 2 ** 32 - 1], dtype=np.uint32)
+    n = len(seeds)
+    diff = np.full(n, 0.8)
+    ids = np.arange(n, dtype=np.int32)
+    g = native.Sim(n)
+    o = P.Lib("o32").sim(n)
+    g.reset(ids, seeds, diff)
+    o.reset(ids, seeds, diff)
+    for i in range(n):
+        assert np.array_equal(g.terrain(i), o.terrain(i)), f"terrain of seed {seeds[i]}"
+    t = action_stream(n, 1, seed=3)[0]
+    g.step(t)
+    o.step(t)
+    a, b = _states(g, n), _states(o, n)
+    assert np.all(np.abs(a - b) <= REL * (np.abs(b) + 1.0))
 
 
 @pytest.mark.parametrize("difficulty", [0.0, 0.5, "mixed"])
