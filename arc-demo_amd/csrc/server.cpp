@@ -324,7 +324,7 @@ int arcs_used(const arc_server *cs)
 
 int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficulty)
 {
-    if (n_worlds <= 0)
+    if (n_worlds <= 0 || difficulty > MAX_DIFFICULTY) /* beyond the device terrain's range */
         return -1;
     int id = -1, start = -1;
     {

@@ -32,6 +32,8 @@ def test_admission_by_budget(mods):
     assert sv.used == sv.capacity - 600
     c = sv.open(500)
     assert sv.range(c)[0] == 0 and sv.range(b)[0] == 600
+    with pytest.raises(server.AdmissionError):  # beyond the device terrain's range
+        sv.open(10, difficulty=7.0)
     sv.close_server()
 
 
