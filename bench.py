@@ -137,30 +137,38 @@ def kernel_pass(steps: int, warm: int) -> dict:
 
 
 def forms_pass(steps: int, warm: int) -> dict:
-    """the step kernel of each form at 65,536 worlds from the same settled
-    state (all forms compute the same bits): HIP events around each launch"""
+    """the step kernel of each form at 65,536 worlds (top-level keys) and at
+    4,096 (under "worlds_4096"), each from a settled state reached in the leg
+    form (all forms compute the same bits): HIP events around each launch"""
     from arc import native
-    shapes = {"lane": 64, "leg": 64, "pair": 64, "shared": 256}
+    shapes = {"lane": 64, "leg": 64, "pair": 64, "shared": 256, "wave": 64}
     out = {}
     S = pool_sets(warm, steps)
-    t = native.DeviceBuffer.from_array(target_pool(native.Sim(1).default_angles, N, S))
-    per = N * NJ
-    for form, block in shapes.items():
-        s = new_sim(N)
-        s.set_form("leg")
-        for k in range(warm):
-            s.step_batch([0], [N], t.ptr + 4 * per * (k % S), task=True, graph=True)
-        s.sync()
-        s.set_form(form)
-        s.set_launch(block, 0)
-        s.set_layout(16)
-        kms = 0.0
-        for k in range(steps):
-            m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
-            kms += m
-        out[form] = {"launch": s.launch(), "kernel_ms_avg": kms / steps}
-        s.close()
-    t.close()
+    defaults = native.Sim(1).default_angles
+    for n in (N, 4096):
+        t = native.DeviceBuffer.from_array(target_pool(defaults, n, S))
+        per = n * NJ
+        res = {}
+        for form, block in shapes.items():
+            s = new_sim(n)
+            s.set_form("leg")
+            for k in range(warm):
+                s.step_batch([0], [n], t.ptr + 4 * per * (k % S), task=True, graph=True)
+            s.sync()
+            s.set_form(form)
+            s.set_launch(block, 0)
+            s.set_layout(16)
+            kms = 0.0
+            for k in range(steps):
+                m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
+                kms += m
+            res[form] = {"launch": s.launch(), "kernel_ms_avg": kms / steps}
+            s.close()
+        t.close()
+        if n == N:
+            out.update(res)
+        else:
+            out[f"worlds_{n}"] = res
     return out
 
 
