@@ -436,10 +436,11 @@ int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out)
                     ok = ok && hipEventSynchronize(e1) == hipSuccess;
                     ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
                     ms /= (float)steps;
-                    /* the persistent distribution only when clearly faster: it pays where the
-                     * ranges are many and small, and otherwise wins or loses by run-to-run
-                     * noise while adding a counter reset to every step */
-                    if (ok && ms < best_ms * (pi ? 0.97 : 1.0)) {
+                    /* the persistent distribution only when far faster: it pays where the
+                     * ranges are many and small, while this one full range has timed it a few
+                     * per cent faster than it then ran in steady serving (graph replays,
+                     * a counter reset every step) */
+                    if (ok && ms < best_ms * (pi ? 0.9 : 1.0)) {
                         best_ms = ms;
                         best[0] = s->block;
                         best[1] = s->tile;

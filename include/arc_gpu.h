@@ -137,7 +137,7 @@ int arcx_graph_captures(const arc_sim *s);
  * block; LDS layout; distribution) over `steps` control steps of this
  * handle's own worlds from their current state (restored afterwards), with
  * fresh uniform actions at every step, keeps the fastest (a persistent
- * distribution only when at least 3 % faster than the best grid-stride one)
+ * distribution only when at least 10 % faster than the best grid-stride one)
  * and reports it in out5 (as arcx_get_launch) and its ms per step in ms_out. */
 int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out);
 
