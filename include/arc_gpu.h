@@ -96,7 +96,7 @@ double arcx_bytes_per_world(const arc_sim *s);
 /* Contact-list overflow flags of world i (0 expected). */
 int arcx_world_error(const arc_sim *s, int i);
 
-/* ---- batched, persistent and graph form (lane and leg forms) ---------
+/* ---- batched, persistent and graph form (register-resident forms) ---
  * No counterpart in the reference, whose arc_sim_step always steps every
  * world (csrc/arc_sim.c:123-138); these serve callers that step several
  * independent requests' worlds together. */
