@@ -56,7 +56,8 @@ long long arcs_submit(arc_server *sv, int client, const float *targets);
  * completely before the call (the caller synchronises its producer) and left
  * unchanged until arcs_wait on the ticket returns.  The step kernels read them
  * where they lie (no copy; nothing crosses PCIe).
- * Returns a ticket > 0, or -1.  Requests of either kind may be mixed. */
+ * Returns a ticket > 0, or -1 (also when targets_dev is not device memory of
+ * the server's GPU).  Requests of either kind may be mixed. */
 long long arcs_submit_device(arc_server *sv, int client, const float *targets_dev);
 /* Blocks until the ticket's step is done; returns its latency in
  * milliseconds (submit to completion), or -1 for an unknown ticket. */

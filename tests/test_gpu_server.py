@@ -115,6 +115,9 @@ def test_device_submission_matches_host(mods):
     torch.cuda.synchronize()
     with pytest.raises(ValueError):
         sv.submit_device(b, last[:-1])
+    host = np.ascontiguousarray(acts[0])
+    with pytest.raises(RuntimeError):  # a host address is refused before any kernel could read it
+        sv.submit_device(b, host.ctypes.data)
     ta = [sv.submit(a, acts[k]) for k in range(steps)]
     tb = [sv.submit_device(b, x) for x in bufs] + [sv.submit_device(b, last)]
     for t in ta + tb:
