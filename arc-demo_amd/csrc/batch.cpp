@@ -274,6 +274,10 @@ static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_re
             set_error(s, -1, "arcx_step_batch_rows: a request without targets", hipSuccess);
             return -1;
         }
+    if (targets_dev && !device_memory(s, targets_dev)) {
+        set_error(s, -1, "arcx_step_batch: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
     /* ranges inside [0, N) and disjoint: overlapping ranges would race */
     std::vector<std::pair<int, int>> iv;
     iv.reserve(n_req);

@@ -463,12 +463,8 @@ long long arcs_submit_device(arc_server *sv, int client, const float *targets_de
     long long ticket;
     /* the step kernels will read this address: it must be memory of the
      * server's device (a host address would fault the launch) */
-    hipPointerAttribute_t pa{};
-    if (!targets_dev || hipPointerGetAttributes(&pa, targets_dev) != hipSuccess
-        || !(pa.type == hipMemoryTypeDevice || pa.type == hipMemoryTypeManaged) || pa.device != sv->sim->device) {
-        (void)hipGetLastError(); /* clear the lookup's error, if any */
+    if (!device_memory(sv->sim, targets_dev))
         return -1;
-    }
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open || !targets_dev)

@@ -76,6 +76,18 @@ static bool ensure_pin(arc_sim *s, size_t bytes)
     return true;
 }
 
+/* memory of this sim's GPU: the step kernels read targets there, and a host
+ * address would fault the launch */
+bool device_memory(const arc_sim *s, const void *p)
+{
+    hipPointerAttribute_t a{};
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError(); /* clear the lookup's error */
+        return false;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == s->device;
+}
+
 /* the configured form of the step kernel over all worlds */
 hipError_t launch_form(arc_sim *s, const float *tf)
 {
@@ -502,6 +514,10 @@ int arcx_terrain(const arc_sim *cs, int i, double *out)
 
 int arcx_step_device(arc_sim *s, const float *targets_dev)
 {
+    if (!device_memory(s, targets_dev)) {
+        set_error(s, -1, "arcx_step_device: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
     HIPCHK_RET(s, launch_form(s, targets_dev), "arcx_step_device", -1);
     return 0;
 }
@@ -511,6 +527,10 @@ int arcx_step_device(arc_sim *s, const float *targets_dev)
 int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long step_floats, double *kernel_ms,
                     double *wall_ms)
 {
+    if (!device_memory(s, targets_dev)) {
+        set_error(s, -1, "arcx_time_steps: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
     hipEvent_t e0, e1, t0, t1;
     HIPCHK_RET(s, hipEventCreate(&e0), "arcx_time_steps", -1);
     HIPCHK_RET(s, hipEventCreate(&e1), "arcx_time_steps", -1);
@@ -554,6 +574,10 @@ int arcx_task_config(arc_sim *s, int beached_steps, double flip_up_z, double voi
 /* step, fall rules, auto-reset of the fallen worlds: no host round trip */
 int arcx_step_task(arc_sim *s, const float *targets_dev)
 {
+    if (!device_memory(s, targets_dev)) {
+        set_error(s, -1, "arcx_step_task: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
     HIPCHK_RET(s, launch_fill_int(s->ts.count, 0, 1, s->stream), "arcx_step_task", -1);
     HIPCHK_RET(s, launch_form(s, targets_dev), "arcx_step_task step", -1);
     HIPCHK_RET(s, launch_falls(s->dev, s->d_model, s->task, s->ts, nullptr, s->stream), "arcx_step_task falls", -1);

@@ -64,6 +64,8 @@ void batch_free(arc_sim *s);
 hipError_t launch_lane_all(arc_sim *s, const float *targets);
 /* the leg form's contact lists (allocated on first use) */
 hipError_t ensure_leg(arc_sim *s);
+/* p is memory of the sim's GPU (device or managed) */
+bool device_memory(const arc_sim *s, const void *p);
 /* buffers the configured launch allocates on first use, made before a capture */
 hipError_t prepare_launch(arc_sim *s);
 

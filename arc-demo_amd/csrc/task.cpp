@@ -274,6 +274,10 @@ int arct_step(arc_task *t, const float *actions_dev, int graph)
 {
     arc_sim *s = t->sim;
     const float *a = actions_dev ? actions_dev : t->actions;
+    if (actions_dev && (!graph || !t->exec || t->graph_actions != a) && !device_memory(s, actions_dev)) {
+        set_error(s, -1, "arct_step: actions are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
     if (!graph) {
         HIPCHK_RET(s, enqueue_step(t, a), "arct_step", -1);
         return 0;

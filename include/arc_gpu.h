@@ -47,7 +47,11 @@ int arcx_state(const arc_sim *s, int i, double *out);
 int arcx_terrain(const arc_sim *s, int i, double *out);
 
 /* Device-resident stepping: targets_dev is an N x 12 float32 device array
- * (world-major).  Asynchronous on the handle's stream; arcx_sync waits. */
+ * (world-major).  Asynchronous on the handle's stream; arcx_sync waits.
+ * The entry points that take one device address (these, arcx_time_steps,
+ * arcx_step_task, arcx_step_batch) check that it is memory of the sim's GPU
+ * and return -1 with a sticky error otherwise, before any kernel could read
+ * it; arcx_step_batch_rows leaves that check to its caller. */
 int arcx_step_device(arc_sim *s, const float *targets_dev);
 int arcx_sync(arc_sim *s);
 

@@ -3,6 +3,8 @@
 - non-finite targets: NaN passes the reference's clamp and makes its world
   non-finite and unstable (frozen from then on), +-inf clamp to the joint
   limits; every form against the oracle (NaN == NaN), flags exact;
+- host addresses where device targets are expected: refused before any
+  kernel could read them, the sim left as it was;
 - arc_sim_observe with any subset of its outputs (the others NULL, as the
   reference allows): each requested output equals the full read-out's, and
   the buffers not requested are left untouched;
@@ -72,6 +74,23 @@ def test_non_finite_targets(native):
             assert same(a, want[k][0]).all(), f"{form}, step {k + 1}"
             assert np.array_equal(flags, want[k][1]), f"{form}, step {k + 1}"
         g.close()
+
+
+def test_host_addresses_refused(native):
+    n = 8
+    g = native.Sim(n)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), np.full(n, 0.4))
+    before = _states(g, n)
+    host = np.ascontiguousarray(action_stream(n, 1)[0].astype(np.float32))
+    for call in (lambda: g.step_device(host.ctypes.data), lambda: g.step_task(host.ctypes.data),
+                 lambda: g.time_steps(host.ctypes.data, 1, 0),
+                 lambda: g.step_batch([0], [n], host.ctypes.data)):
+        with pytest.raises(native.SimError):
+            call()
+    g.sync()
+    assert same(_states(g, n), before).all()
+    g.close()
 
 
 def test_observe_output_subsets(native):
