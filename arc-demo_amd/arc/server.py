@@ -60,7 +60,6 @@ class StepServer:
         self._ptr = self._lib.arcs_create(float(hbm_budget_bytes), dt, substeps, grid, cell)
         if not self._ptr:
             raise RuntimeError("arcs_create failed (no GPU, or the budget holds no world)")
-        self._sizes: dict[int, int] = {}
 
     @property
     def capacity(self) -> int:
@@ -74,13 +73,16 @@ class StepServer:
         cid = self._lib.arcs_open(self._ptr, n_worlds, seed0, difficulty)
         if cid < 0:
             raise AdmissionError(f"{n_worlds} worlds do not fit ({self.used} of {self.capacity} held)")
-        self._sizes[cid] = n_worlds
         return cid
 
     def close(self, cid: int) -> None:
         if self._lib.arcs_close(self._ptr, cid):
             raise KeyError(cid)
-        self._sizes.pop(cid, None)
+
+    def _count(self, cid: int) -> int:
+        """World count of an open client, read from the server (which owns the
+        client table) so that no Python-side copy can go stale."""
+        return self.range(cid)[1]
 
     def range(self, cid: int) -> tuple[int, int]:
         s, c = _int(), _int()
@@ -90,8 +92,9 @@ class StepServer:
 
     def submit(self, cid: int, targets: np.ndarray) -> int:
         t = np.ascontiguousarray(targets, dtype=np.float32)
-        if t.size != self._sizes[cid] * 12:
-            raise ValueError(f"client {cid} needs {self._sizes[cid]} x 12 targets")
+        n = self._count(cid)
+        if t.size != n * 12:
+            raise ValueError(f"client {cid} needs {n} x 12 targets")
         ticket = self._lib.arcs_submit(self._ptr, cid, t.ctypes.data)
         if ticket < 0:
             raise RuntimeError("arcs_submit failed")
@@ -102,16 +105,17 @@ class StepServer:
         float32 tensor on the GPU (n x 12 elements, contiguous), a
         native.DeviceBuffer, or a raw device address.  The memory must be
         written before the call and left unchanged until wait(ticket)."""
-        n = self._sizes[cid] * 12
+        w = self._count(cid)
+        n = w * 12
         if hasattr(targets, "data_ptr"):
             if not targets.is_cuda or str(targets.dtype) != "torch.float32" or not targets.is_contiguous():
                 raise ValueError("device targets must be a contiguous float32 GPU tensor")
             if targets.numel() != n:
-                raise ValueError(f"client {cid} needs {self._sizes[cid]} x 12 targets")
+                raise ValueError(f"client {cid} needs {w} x 12 targets")
             ptr = targets.data_ptr()
         elif hasattr(targets, "ptr"):
             if getattr(targets, "nbytes", 4 * n) < 4 * n:
-                raise ValueError(f"client {cid} needs {self._sizes[cid]} x 12 targets")
+                raise ValueError(f"client {cid} needs {w} x 12 targets")
             ptr = targets.ptr
         else:
             ptr = int(targets)
@@ -150,7 +154,7 @@ class StepServer:
             raise ValueError(microseconds)
 
     def observe(self, cid: int) -> np.ndarray:
-        out = np.zeros((self._sizes[cid], 13))
+        out = np.zeros((self._count(cid), 13))
         if self._lib.arcs_observe(self._ptr, cid, out.ctypes.data):
             raise RuntimeError("arcs_observe failed")
         return out

@@ -17,6 +17,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <limits.h>
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -73,7 +74,10 @@ struct arc_server {
     std::condition_variable landed; /* a flight was launched or completed */
     std::thread completer;
     long long launched = 0;
-    std::vector<Client> clients;
+    /* admitted clients by id; ids are issued once and never reused, so a
+     * caller's id cannot come to name another client after a close */
+    std::map<int, Client> clients;
+    int next_client = 0;
     std::vector<std::pair<int, int>> free_ranges; /* (start, count), sorted by start */
 
     std::mutex mu;                 /* queues, clients, tickets, stats */
@@ -123,9 +127,9 @@ static void dispatch_loop(arc_server *sv)
             std::unique_lock<std::mutex> lk(sv->mu);
             auto pending = [&](bool all) {
                 int open = 0, ready = 0;
-                for (auto &c : sv->clients) {
-                    open += c.open || !c.queue.empty();
-                    ready += !c.queue.empty();
+                for (auto &kv : sv->clients) {
+                    open += kv.second.open || !kv.second.queue.empty();
+                    ready += !kv.second.queue.empty();
                 }
                 return all ? ready > 0 && ready == open : ready > 0;
             };
@@ -133,21 +137,21 @@ static void dispatch_loop(arc_server *sv)
             if (!sv->stop && !pending(true)) {
                 /* gather more requests for up to the window after the oldest */
                 clock_type::time_point first = clock_type::time_point::max();
-                for (auto &c : sv->clients)
-                    if (!c.queue.empty())
-                        first = std::min(first, c.queue.front().submitted);
+                for (auto &kv : sv->clients)
+                    if (!kv.second.queue.empty())
+                        first = std::min(first, kv.second.queue.front().submitted);
                 sv->work.wait_until(lk, first + sv->window, [&] { return sv->stop || pending(true); });
             }
             if (sv->stop && !pending(false))
                 break;
             /* a free targets buffer: at most DEPTH flights */
             sv->landed.wait(lk, [&] { return (int)sv->flights.size() < DEPTH; });
-            for (auto &c : sv->clients)
-                if (!c.queue.empty()) {
+            for (auto &kv : sv->clients)
+                if (!kv.second.queue.empty()) {
+                    Client &c = kv.second;
                     f.reqs.push_back(std::move(c.queue.front()));
                     c.queue.pop_front();
-                    /* the range as it is now: the client table may grow (and move) once
-                     * the lock is released, and a client's range is fixed while it has
+                    /* the range as it is now: a client's range is fixed while it has
                      * requests queued or in flight */
                     f.reqs.back().start = c.start;
                     f.reqs.back().count = c.count;
@@ -329,7 +333,7 @@ int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficult
     int id = -1, start = -1;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
-        if (sv->used + n_worlds > sv->capacity)
+        if (sv->used + n_worlds > sv->capacity || sv->next_client == INT_MAX)
             return -1;
         for (auto &r : sv->free_ranges)
             if (r.second >= n_worlds) {
@@ -343,17 +347,7 @@ int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficult
         sv->free_ranges.erase(std::remove_if(sv->free_ranges.begin(), sv->free_ranges.end(),
                                              [](const std::pair<int, int> &r) { return r.second == 0; }),
                               sv->free_ranges.end());
-        /* a slot is reused once its client is closed and its range released
-         * (count 0): a closing client keeps its slot until then */
-        for (size_t k = 0; k < sv->clients.size(); ++k)
-            if (!sv->clients[k].open && sv->clients[k].queue.empty() && sv->clients[k].count == 0) {
-                id = (int)k;
-                break;
-            }
-        if (id < 0) {
-            id = (int)sv->clients.size();
-            sv->clients.emplace_back();
-        }
+        id = sv->next_client++;
         Client &c = sv->clients[id];
         c.start = start;
         c.count = n_worlds;
@@ -392,22 +386,22 @@ int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficult
 int arcs_close(arc_server *sv, int client)
 {
     std::unique_lock<std::mutex> lk(sv->mu);
-    if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open)
+    auto it = sv->clients.find(client);
+    if (it == sv->clients.end() || !it->second.open)
         return -1;
-    sv->clients[client].open = false; /* no new submissions */
-    /* the table is indexed afresh after the wait: arcs_open may grow it meanwhile */
+    Client &c = it->second; /* map nodes stay put while other clients come and go */
+    c.open = false;         /* no new submissions */
     sv->done.wait(lk, [&] {
-        if (!sv->clients[client].queue.empty())
+        if (!c.queue.empty())
             return false;
         for (auto &kv : sv->inflight)
             if (kv.second == client)
                 return false;
         return true;
     });
-    Client &c = sv->clients[client];
     release_range(sv, c.start, c.count);
     sv->used -= c.count;
-    c.count = 0;
+    sv->clients.erase(it);
     return 0;
 }
 
@@ -415,10 +409,11 @@ int arcs_client_range(const arc_server *cs, int client, int *start, int *count)
 {
     arc_server *sv = const_cast<arc_server *>(cs);
     std::lock_guard<std::mutex> lk(sv->mu);
-    if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open)
+    auto it = sv->clients.find(client);
+    if (it == sv->clients.end() || !it->second.open)
         return -1;
-    *start = sv->clients[client].start;
-    *count = sv->clients[client].count;
+    *start = it->second.start;
+    *count = it->second.count;
     return 0;
 }
 
@@ -430,9 +425,10 @@ long long arcs_submit(arc_server *sv, int client, const float *targets)
     size_t nf;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
-        if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open || !targets)
+        auto it = sv->clients.find(client);
+        if (it == sv->clients.end() || !it->second.open || !targets)
             return -1;
-        nf = (size_t)sv->clients[client].count * NJ;
+        nf = (size_t)it->second.count * NJ;
         if (!sv->pool.empty()) {
             r.targets = std::move(sv->pool.back());
             sv->pool.pop_back();
@@ -442,11 +438,12 @@ long long arcs_submit(arc_server *sv, int client, const float *targets)
     r.targets.assign(targets, targets + nf);
     {
         std::lock_guard<std::mutex> lk(sv->mu);
-        Client &c = sv->clients[client];
-        if (!c.open || (size_t)c.count * NJ != nf) {
+        auto it = sv->clients.find(client); /* closed meanwhile? */
+        if (it == sv->clients.end() || !it->second.open || (size_t)it->second.count * NJ != nf) {
             sv->pool.push_back(std::move(r.targets));
             return -1;
         }
+        Client &c = it->second;
         r.ticket = ticket = sv->next_ticket++;
         r.client = client;
         r.submitted = t;
@@ -467,7 +464,8 @@ long long arcs_submit_device(arc_server *sv, int client, const float *targets_de
         return -1;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
-        if (client < 0 || client >= (int)sv->clients.size() || !sv->clients[client].open || !targets_dev)
+        auto it = sv->clients.find(client);
+        if (it == sv->clients.end() || !it->second.open || !targets_dev)
             return -1;
         Request r;
         r.dev = targets_dev;
@@ -475,7 +473,7 @@ long long arcs_submit_device(arc_server *sv, int client, const float *targets_de
         r.client = client;
         r.submitted = t;
         sv->inflight[ticket] = client;
-        sv->clients[client].queue.push_back(std::move(r));
+        it->second.queue.push_back(std::move(r));
     }
     sv->work.notify_one();
     return ticket;

@@ -41,7 +41,9 @@ int arcs_used(const arc_server *sv);
 /* Admission: n_worlds worlds reset with seeds seed0 + k and the given
  * difficulty (< 0: per world uniform in [0, 1) from the engine RNG seeded
  * seed0).  Returns a client id >= 0, or -1 when the budget or the free space
- * cannot hold them or the difficulty exceeds 6 (nothing is changed then). */
+ * cannot hold them or the difficulty exceeds 6 (nothing is changed then).
+ * Ids are issued in increasing order and never reused, so an id held after
+ * its client was closed names no other client. */
 int arcs_open(arc_server *sv, int n_worlds, unsigned int seed0, double difficulty);
 /* Releases a client's worlds after its pending requests are stepped. */
 int arcs_close(arc_server *sv, int client);

@@ -30,8 +30,13 @@ def test_admission_by_budget(mods):
     assert sv.used == sv.capacity
     sv.close(a)
     assert sv.used == sv.capacity - 600
+    with pytest.raises(KeyError):  # a closed client's id stays unknown
+        sv.range(a)
     c = sv.open(500)
+    assert c not in (a, b)  # ids are never reused, so a stale id cannot name c
     assert sv.range(c)[0] == 0 and sv.range(b)[0] == 600
+    with pytest.raises(KeyError):
+        sv.range(a)
     with pytest.raises(server.AdmissionError):  # beyond the device terrain's range
         sv.open(10, difficulty=7.0)
     sv.close_server()
