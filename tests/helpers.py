@@ -35,6 +35,58 @@ def same(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return (a == b) | (np.isnan(a) & np.isnan(b))
 
 
+# smallest normal binary32: differences below it are denormal noise around an
+# exact zero, where a relative bound means nothing
+FLOOR32 = float(np.finfo(np.float32).tiny)
+
+
+def rel_close(a, b, rel: float = 1e-5, floor: float = FLOOR32) -> np.ndarray:
+    """|a - b| <= rel |b| elementwise (pure relative), with an absolute floor
+    at the smallest normal binary32 for values at or next to zero; NaN equals
+    NaN, and an infinity only itself."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    with np.errstate(invalid="ignore"):
+        d = np.abs(a - b)
+        return same(a, b) | (d <= rel * np.abs(b)) | (d <= floor)
+
+
+def max_rel(a, b) -> float:
+    """Largest |a - b| / |b| over the finite pairs that differ (0 if none)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    k = ~same(a, b) & np.isfinite(a) & np.isfinite(b)
+    if not k.any():
+        return 0.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return float(np.max(np.abs(a[k] - b[k]) / np.maximum(np.abs(b[k]), FLOOR32)))
+
+
+def _first_diff(ok: np.ndarray, a, b) -> str:
+    bad = np.argwhere(~ok)
+    i = tuple(bad[0])
+    return f"{len(bad)} of {ok.size} values differ; first at {i}: {a[i]!r} vs {b[i]!r}; max rel {max_rel(a, b):.3g}"
+
+
+def assert_same(a, b, what: str = "") -> None:
+    """Bit-exact equality (NaN == NaN, -0 == +0), with the first difference
+    and the largest relative error in the message."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shapes {a.shape} vs {b.shape}"
+    ok = same(a, b)
+    assert ok.all(), f"{what}: {_first_diff(ok, a, b)}"
+
+
+def assert_rel(a, b, rel: float = 1e-5, what: str = "") -> None:
+    """rel_close everywhere, with the first violation in the message."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    assert a.shape == b.shape, f"{what}: shapes {a.shape} vs {b.shape}"
+    ok = rel_close(a, b, rel)
+    assert ok.all(), f"{what}: {_first_diff(ok, a, b)}"
+
+
 class HostCheck:
     """The device engine's code compiled for the host (tests/hostcheck)."""
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import pyoracle as P
-from helpers import action_stream
+from helpers import action_stream, assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -57,7 +57,7 @@ def test_batch_steps_only_its_ranges(native):
     o.step(t.astype(np.float64))
     a = after[inside].view(np.float64)
     b = np.stack([o.state(i) for i in np.flatnonzero(inside)])
-    assert np.all(np.abs(a - b) <= 1e-5 * (np.abs(b) + 1.0))
+    assert_same(a, b, "batched ranges vs oracle")
 
 
 @pytest.mark.parametrize("form", ["lane", "leg"])

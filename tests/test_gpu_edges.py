@@ -12,8 +12,8 @@
   requests do nothing and report no error (the reference's loops are empty);
 - world counts that fill no whole wavefront or quad group (1, 3, 17, 100):
   every form (lane, four lanes per world, two lanes per world, four lanes per
-  world with shared root solves, one world per wavefront) against the FP32
-  oracle after one step, and bit-identical to each other;
+  world with shared root solves, one world per wavefront) bit-exact with the
+  FP32 oracle after one step;
 - reset inputs: an empty list (nothing changes), repeated ids (the last entry
   wins, as resetting in list order does), ids out of range (skipped, the
   others reset, the error reported), difficulties beyond the int8 terrain's
@@ -31,12 +31,9 @@ import numpy as np
 import pytest
 
 import pyoracle as P
-from helpers import action_stream, same
+from helpers import action_stream, assert_same, same
 
 pytestmark = pytest.mark.gpu
-
-REL = 1e-5
-
 
 @pytest.fixture(scope="module")
 def native():
@@ -146,7 +143,7 @@ def test_ragged_world_counts_all_forms(native, n):
         g.reset(ids, seeds, diff)
         g.step(t)
         a = _states(g, n)
-        assert np.all(np.abs(a - b) <= REL * (np.abs(b) + 1.0)), f"{form}, {n} worlds"
+        assert_same(a, b, f"{form}, {n} worlds vs oracle")
         if ref is None:
             ref = a
         else:

@@ -5,13 +5,13 @@ Falls are rare under random actions (0.2-1.3 per 1000 world-steps), so the
 test tightens the rules (flip when the torso's up axis tilts past z < 0.97,
 beached after 2 steps of torso contact) to make many worlds terminate and
 reset within a few steps.  Causes and reset seeds (seed = i + N * episode)
-are integers and must agree exactly; states within the one-step bar."""
+are integers and must agree exactly; states bit for bit."""
 
 import numpy as np
 import pytest
 
 import pyoracle as P
-from helpers import action_stream
+from helpers import action_stream, assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -51,7 +51,7 @@ def test_falls_and_autoreset_match_oracle(form):
             o.reset(fell.astype(np.int32), seeds, diff[fell])
         a = np.stack([g.state(i) for i in range(n)])
         b = np.stack([o.state(i) for i in range(n)])
-        assert np.all(np.abs(a - b) <= 1e-4 * (np.abs(b) + 1.0)), f"state after step {k + 1}"
+        assert_same(a, b, f"{form}: state after step {k + 1}")
     assert total > 10, "the tightened rules should terminate many worlds"
     assert np.array_equal(g.task_episodes(), episode.astype(np.uint32))
     stats = g.task_stats()

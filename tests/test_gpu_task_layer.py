@@ -15,23 +15,23 @@ bit-exact between the two, the task arithmetic is binary64 on both sides (only
 the device's and glibc's atan2 / cos / sin / hypot may differ in the last
 bit), so: integers (terminations, causes,
 checkpoint counts, episode steps, reset lists) and curriculum levels exact;
-rewards and observations (float32) within 1e-5 (|b| + 1).  The tightened
-config makes checkpoints, falls, truncations and level changes frequent."""
+rewards and observations (float32) within 1e-5 relative (floored at the
+smallest normal binary32).  The tightened config makes checkpoints, falls,
+truncations and level changes frequent."""
 
 import numpy as np
 import pytest
 
 import pyoracle as P
-from helpers import action_stream
+from helpers import action_stream, rel_close
 from pytask import OracleTask
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-5
-
 
 def _close(a, b):
-    return np.all(np.abs(a.astype(np.float64) - b.astype(np.float64)) <= TOL * (np.abs(b) + 1.0))
+    """pure relative 1e-5, floored at the smallest normal binary32"""
+    return bool(rel_close(a, b, 1e-5).all())
 
 
 def _same_state(t, rng):

@@ -4,23 +4,24 @@ arc/env.py:99-117, on oracle/pytask.py with numpy's generator): observations,
 rewards, dones and the infos of ended worlds (terminal observation, time-limit
 flag, checkpoints, difficulty, fell, cause), seeds and checkpoints from the
 same np.random.default_rng stream, re-seeding through seed() + reset().
-Integers, flags and causes exact; float32 outputs within 1e-5 (|b| + 1)."""
+Integers, flags and causes exact; float32 outputs within 1e-5 relative
+(floored at the smallest normal binary32)."""
 
 import numpy as np
 import pytest
 
 import pyoracle as P
-from helpers import action_stream
+from helpers import action_stream, rel_close
 from pytask import OracleTask
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-5
 CAUSES = ("", "unstable", "void", "flip", "beached")
 
 
 def _close(a, b):
-    return np.all(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) <= TOL * (np.abs(b) + 1.0))
+    """pure relative 1e-5, floored at the smallest normal binary32"""
+    return bool(rel_close(a, b, 1e-5).all())
 
 
 class OracleVecEnv:
