@@ -263,7 +263,7 @@ int arcx_get_launch(const arc_sim *s, int *out5)
 /* arcx_step_batch and arcx_step_batch_rows: request r's targets start at
  * rows[r] (rows given), else at targets_dev + starts[r] x 12 */
 static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev,
-                      const float *const *rows, int task, int graph)
+                      const float *const *rows, int task, int graph, bool rows_checked)
 {
     if (n_req < 0 || n_req > MAX_BATCH || (n_req && (!starts || !counts)) || (!targets_dev && !rows)) {
         set_error(s, -1, "arcx_step_batch: request list", hipSuccess);
@@ -274,21 +274,32 @@ static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_re
             set_error(s, -1, "arcx_step_batch_rows: a request without targets", hipSuccess);
             return -1;
         }
-    if (targets_dev && !device_memory(s, targets_dev)) {
-        set_error(s, -1, "arcx_step_batch: targets are not device memory of the sim's GPU", hipSuccess);
-        return -1;
-    }
     /* ranges inside [0, N) and disjoint: overlapping ranges would race */
     std::vector<std::pair<int, int>> iv;
     iv.reserve(n_req);
+    size_t end = 0; /* targets_dev must hold the rows up to the last range's end */
     for (int r = 0; r < n_req; ++r) {
         if (counts[r] < 0 || starts[r] < 0 || starts[r] > s->n - counts[r]) {
             set_error(s, -1, "arcx_step_batch: range outside the sim", hipSuccess);
             return -1;
         }
-        if (counts[r])
+        if (counts[r]) {
             iv.emplace_back(starts[r], counts[r]);
+            end = std::max(end, (size_t)starts[r] + (size_t)counts[r]);
+        }
     }
+    if (targets_dev && !device_memory(s, targets_dev, end * NJ * sizeof(float))) {
+        set_error(s, -1, "arcx_step_batch: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
+    /* every request's row block: counts[r] x 12 floats inside one allocation
+     * of the sim's GPU (a host address would fault the kernels) */
+    for (int r = 0; rows && !rows_checked && r < n_req; ++r)
+        if (counts[r] > 0 && !device_memory(s, rows[r], (size_t)counts[r] * NJ * sizeof(float))) {
+            set_error(s, -1, "arcx_step_batch_rows: a request's targets are not device memory of the sim's GPU",
+                      hipSuccess);
+            return -1;
+        }
     std::sort(iv.begin(), iv.end());
     for (size_t k = 1; k < iv.size(); ++k)
         if (iv[k - 1].first + iv[k - 1].second > iv[k].first) {
@@ -354,14 +365,24 @@ static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_re
 int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req, const float *targets_dev, int task,
                     int graph)
 {
-    return step_batch(s, starts, counts, n_req, targets_dev, nullptr, task, graph);
+    return step_batch(s, starts, counts, n_req, targets_dev, nullptr, task, graph, false);
 }
 
 int arcx_step_batch_rows(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
                          int task, int graph)
 {
-    return step_batch(s, starts, counts, n_req, nullptr, rows, task, graph);
+    return step_batch(s, starts, counts, n_req, nullptr, rows, task, graph, false);
 }
+
+} /* extern "C" */
+
+int step_batch_rows_checked(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
+                            int task, int graph)
+{
+    return step_batch(s, starts, counts, n_req, nullptr, rows, task, graph, true);
+}
+
+extern "C" {
 
 int arcx_graph_captures(const arc_sim *s) { return s->batch ? s->batch->captures : 0; }
 

@@ -184,8 +184,10 @@ static void dispatch_loop(arc_server *sv)
             if (uploads)
                 f.ok = f.ok && hipEventRecord(sv->copied[f.buf], sv->copy) == hipSuccess
                        && hipStreamWaitEvent(s->stream, sv->copied[f.buf], 0) == hipSuccess;
-            f.ok = f.ok && arcx_step_batch_rows(s, starts.data(), counts.data(), (int)f.reqs.size(), rows.data(), 1, 1)
-                               == 0;
+            /* rows: the staging buffers, or device targets checked at submission */
+            f.ok = f.ok
+                   && step_batch_rows_checked(s, starts.data(), counts.data(), (int)f.reqs.size(), rows.data(), 1, 1)
+                          == 0;
             f.ok = f.ok && hipEventRecord(sv->ev[f.buf], s->stream) == hipSuccess;
         }
         {
@@ -458,14 +460,15 @@ long long arcs_submit_device(arc_server *sv, int client, const float *targets_de
 {
     const auto t = clock_type::now();
     long long ticket;
-    /* the step kernels will read this address: it must be memory of the
-     * server's device (a host address would fault the launch) */
-    if (!device_memory(sv->sim, targets_dev))
-        return -1;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         auto it = sv->clients.find(client);
         if (it == sv->clients.end() || !it->second.open || !targets_dev)
+            return -1;
+        /* the step kernels will read the client's n x 12 floats here: they must
+         * lie in memory of the server's device (a host address would fault
+         * the launch) */
+        if (!device_memory(sv->sim, targets_dev, (size_t)it->second.count * NJ * sizeof(float)))
             return -1;
         Request r;
         r.dev = targets_dev;

@@ -76,16 +76,28 @@ static bool ensure_pin(arc_sim *s, size_t bytes)
     return true;
 }
 
-/* memory of this sim's GPU: the step kernels read targets there, and a host
- * address would fault the launch */
-bool device_memory(const arc_sim *s, const void *p)
+/* [p, p + bytes) lies in one allocation of this sim's GPU: the step kernels
+ * read targets there, and a host address, or a range that runs past the end
+ * of its allocation, would fault the launch */
+bool device_memory(const arc_sim *s, const void *p, size_t bytes)
 {
     hipPointerAttribute_t a{};
     if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError(); /* clear the lookup's error */
         return false;
     }
-    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == s->device;
+    if (!(a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) || a.device != s->device)
+        return false;
+    if (bytes == 0)
+        return true;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t lo = (uintptr_t)base, at = (uintptr_t)p;
+    return at >= lo && bytes <= size - (at - lo);
 }
 
 /* the configured form of the step kernel over all worlds */
@@ -514,7 +526,7 @@ int arcx_terrain(const arc_sim *cs, int i, double *out)
 
 int arcx_step_device(arc_sim *s, const float *targets_dev)
 {
-    if (!device_memory(s, targets_dev)) {
+    if (!device_memory(s, targets_dev, (size_t)s->n * NJ * sizeof(float))) {
         set_error(s, -1, "arcx_step_device: targets are not device memory of the sim's GPU", hipSuccess);
         return -1;
     }
@@ -527,7 +539,10 @@ int arcx_step_device(arc_sim *s, const float *targets_dev)
 int arcx_time_steps(arc_sim *s, const float *targets_dev, int steps, long long step_floats, double *kernel_ms,
                     double *wall_ms)
 {
-    if (!device_memory(s, targets_dev)) {
+    if (steps < 0 || step_floats < 0
+        || !device_memory(s, targets_dev,
+                          ((size_t)(steps > 0 ? steps - 1 : 0) * (size_t)step_floats + (size_t)s->n * NJ)
+                              * sizeof(float))) {
         set_error(s, -1, "arcx_time_steps: targets are not device memory of the sim's GPU", hipSuccess);
         return -1;
     }
@@ -574,7 +589,7 @@ int arcx_task_config(arc_sim *s, int beached_steps, double flip_up_z, double voi
 /* step, fall rules, auto-reset of the fallen worlds: no host round trip */
 int arcx_step_task(arc_sim *s, const float *targets_dev)
 {
-    if (!device_memory(s, targets_dev)) {
+    if (!device_memory(s, targets_dev, (size_t)s->n * NJ * sizeof(float))) {
         set_error(s, -1, "arcx_step_task: targets are not device memory of the sim's GPU", hipSuccess);
         return -1;
     }

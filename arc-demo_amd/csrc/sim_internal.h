@@ -64,8 +64,13 @@ void batch_free(arc_sim *s);
 hipError_t launch_lane_all(arc_sim *s, const float *targets);
 /* the leg form's contact lists (allocated on first use) */
 hipError_t ensure_leg(arc_sim *s);
-/* p is memory of the sim's GPU (device or managed) */
-bool device_memory(const arc_sim *s, const void *p);
+/* [p, p + bytes) lies inside one allocation of the sim's GPU (device or
+ * managed memory); bytes = 0 checks the address only */
+bool device_memory(const arc_sim *s, const void *p, size_t bytes);
+/* arcx_step_batch_rows for rows the caller has already checked with
+ * device_memory (the step server checks them when they are submitted) */
+int step_batch_rows_checked(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
+                            int task, int graph);
 /* buffers the configured launch allocates on first use, made before a capture */
 hipError_t prepare_launch(arc_sim *s);
 

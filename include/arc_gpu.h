@@ -48,10 +48,11 @@ int arcx_terrain(const arc_sim *s, int i, double *out);
 
 /* Device-resident stepping: targets_dev is an N x 12 float32 device array
  * (world-major).  Asynchronous on the handle's stream; arcx_sync waits.
- * The entry points that take one device address (these, arcx_time_steps,
- * arcx_step_task, arcx_step_batch) check that it is memory of the sim's GPU
+ * The entry points that take device addresses (these, arcx_time_steps,
+ * arcx_step_task, arcx_step_batch, arcx_step_batch_rows) check that every
+ * range the kernels will read lies inside one allocation of the sim's GPU
  * and return -1 with a sticky error otherwise, before any kernel could read
- * it; arcx_step_batch_rows leaves that check to its caller. */
+ * it. */
 int arcx_step_device(arc_sim *s, const float *targets_dev);
 int arcx_sync(arc_sim *s);
 
@@ -131,7 +132,8 @@ int arcx_step_batch(arc_sim *s, const int *starts, const int *counts, int n_req,
  * memory, e.g. the output tensor of a client's policy, read where it lies:
  * no copy).  The memory is read when the launch runs and must hold the
  * targets until the handle's stream has passed the step.  Same graphs as
- * arcx_step_batch; -1 if a non-empty request has no rows. */
+ * arcx_step_batch; -1 if a non-empty request has no rows or its rows are not
+ * counts[r] x 12 floats of device memory of the sim's GPU. */
 int arcx_step_batch_rows(arc_sim *s, const int *starts, const int *counts, int n_req, const float *const *rows,
                          int task, int graph);
 /* graphs captured so far by arcx_step_batch */

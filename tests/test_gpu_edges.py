@@ -80,13 +80,27 @@ def test_host_addresses_refused(native):
     g.reset(ids, ids.astype(np.uint32), np.full(n, 0.4))
     before = _states(g, n)
     host = np.ascontiguousarray(action_stream(n, 1)[0].astype(np.float32))
+    # device memory too short for what the kernels would read: 12 floats
+    # short of n x 12 (a whole allocation of libarcsim's, so its extent is
+    # exactly n x 12 - 12 floats)
+    short = native.DeviceBuffer(4 * 12 * (n - 1))
+    full = native.DeviceBuffer.from_array(host)
     for call in (lambda: g.step_device(host.ctypes.data), lambda: g.step_task(host.ctypes.data),
                  lambda: g.time_steps(host.ctypes.data, 1, 0),
-                 lambda: g.step_batch([0], [n], host.ctypes.data)):
+                 lambda: g.step_batch([0], [n], host.ctypes.data),
+                 lambda: g.step_batch_rows([0, 4], [4, 4], [full.ptr, host.ctypes.data]),
+                 lambda: g.step_device(short.ptr), lambda: g.step_task(short.ptr),
+                 lambda: g.time_steps(full.ptr, 2, n * 12),
+                 lambda: g.step_batch([0], [n], short.ptr),
+                 lambda: g.step_batch_rows([0, 4], [4, 4], [full.ptr, short.ptr + 4 * 12 * (n - 4)])):
         with pytest.raises(native.SimError):
             call()
     g.sync()
     assert same(_states(g, n), before).all()
+    # the same entry points with ranges that fit are accepted
+    g.step_batch_rows([0, 4], [4, 4], [full.ptr, short.ptr + 4 * 12 * (n - 5)])
+    g.step_batch([0], [n - 1], short.ptr)
+    g.sync()
     g.close()
 
 
