@@ -32,7 +32,8 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
 # reorders independent instructions only, so the results are the same bits)
 STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
 
-HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip"]
+HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip",
+               "noalloc.hip"]
 HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "task.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp"]
 
 

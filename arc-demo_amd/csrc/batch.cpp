@@ -17,6 +17,7 @@
 #include <utility>
 #include <vector>
 
+#include "noalloc.h"
 #include "sim_internal.h"
 
 namespace {
@@ -232,7 +233,74 @@ hipError_t launch_lane_all(arc_sim *s, const float *targets)
                        is_leg(s), s->stream);
 }
 
+/* bytes of contact lists (and idle-lane lists and anchors of the shared-root
+ * form) a step of every world in the configured register-resident form
+ * writes; the layouts are those of dev_alloc (lane form) and ensure_leg */
+static size_t contact_list_bytes(const arc_sim *s)
+{
+    const Model &M = s->su.model;
+    const size_t S = s->dev.stride;
+    if (!is_leg(s))
+        return (size_t)M.cslots * 6 * S * sizeof(float);
+    const int k = is_leg(s) == 2 ? 2 : 1;
+    size_t floats = (size_t)leg_world_slots(M, k) * 6 * S;
+    if (is_leg(s) == 4)
+        floats += ((size_t)(M.ccap[0] + M.ccap[1] + M.ccap[5]) * 6 + 2 * NS * 3) * 256;
+    return floats * sizeof(float);
+}
+
 extern "C" {
+
+int arcx_step_device_noalloc(arc_sim *s, const float *targets_dev)
+{
+    if (!device_memory(s, targets_dev, (size_t)s->n * NJ * sizeof(float))) {
+        set_error(s, -1, "arcx_step_device_noalloc: targets are not device memory of the sim's GPU", hipSuccess);
+        return -1;
+    }
+    if (s->n > ARCX_NOALLOC_MAX_WORLDS) {
+        set_error(s, -1, "arcx_step_device_noalloc: more worlds than the static scratch holds (8192)", hipSuccess);
+        return -1;
+    }
+    if (s->form == 1) {
+        /* the plain form keeps its contact lists in LDS: no scratch at all */
+        HIPCHK_RET(s, launch_form(s, targets_dev), "arcx_step_device_noalloc", -1);
+        return 0;
+    }
+    if (is_leg(s) && s->su.model.cslots > MAXC) {
+        set_error(s, -1, "arcx_step_device_noalloc: the robot's contact windows exceed the group forms' cap",
+                  hipSuccess);
+        return -1;
+    }
+    char *base = (char *)noalloc_scratch(s->device);
+    if (!base) {
+        set_error(s, -1, "arcx_step_device_noalloc: the static scratch's device address is unknown", hipSuccess);
+        return -1;
+    }
+    const size_t lists = (contact_list_bytes(s) + 255) / 256 * 256;
+    if (lists + 256 > NOALLOC_SCRATCH_BYTES) {
+        set_error(s, -1, "arcx_step_device_noalloc: the sim's contact lists exceed the static scratch", hipSuccess);
+        return -1;
+    }
+    /* the sim's layout with its scratch pointers moved into the static array */
+    DevWorlds d = s->dev;
+    if (is_leg(s)) {
+        d.lcbuf = (float *)base;
+        d.lcdummy = d.lcbuf + (size_t)leg_world_slots(s->su.model, 1) * 6 * d.stride;
+    } else {
+        d.cbuf = (float *)base;
+    }
+    int *work = nullptr;
+    int grid = s->grid_blocks;
+    if (s->persistent) {
+        work = (int *)(base + lists);
+        grid = grid_for(s, s->n);
+        HIPCHK_RET(s, launch_fill_int(work, 0, 1, s->stream), "arcx_step_device_noalloc counter", -1);
+    }
+    HIPCHK_RET(s, launch_step(d, s->d_model, targets_dev, nullptr, s->n, s->block, tile_of(s), grid, work, is_leg(s),
+                              s->stream),
+               "arcx_step_device_noalloc", -1);
+    return 0;
+}
 
 int arcx_set_layout(arc_sim *s, int tile)
 {

@@ -14,6 +14,7 @@
 
 #include "model.h"
 #include "reset_engine.h"
+#include "noalloc.h"
 #include "sim_internal.h"
 
 void set_error(const arc_sim *cs, int code, const char *what, hipError_t e)
@@ -140,6 +141,14 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
     s->tile = 16;
     s->persistent = 0;
     (void)hipGetDevice(&s->device);
+    /* the device address of the allocation-free entry point's static scratch,
+     * looked up once per GPU with that GPU current; a failure here disables
+     * only that entry point (it then reports an error) */
+    {
+        const hipError_t se = noalloc_scratch_init(s->device);
+        if (se != hipSuccess)
+            fprintf(stderr, "libarcsim: arc_sim_create: static scratch lookup failed (%s)\n", hipGetErrorString(se));
+    }
 
     if (!sim_setup(s->su, dt, substeps, grid, cell)) {
         delete s;

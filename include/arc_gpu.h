@@ -56,6 +56,26 @@ int arcx_terrain(const arc_sim *s, int i, double *out);
 int arcx_step_device(arc_sim *s, const float *targets_dev);
 int arcx_sync(arc_sim *s);
 
+/* Allocation-free variant of arcx_step_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It steps every world exactly as arcx_step_device does (same
+ * form, launch shape and distribution, same bits), but the step kernels'
+ * scratch -- the per-substep contact lists and the persistent distribution's
+ * work counter -- lives in one static device array per GPU, held from the
+ * library's load until the process ends, instead of the sim's own buffers;
+ * the call allocates and frees nothing (no hipMalloc, hipFree or allocating
+ * library routine).  Limit: ARCX_NOALLOC_MAX_WORLDS worlds (the array holds
+ * that many worlds' lists in every form); a larger sim is refused with -1
+ * and a sticky error, nothing launched.  Every call on one GPU uses that
+ * GPU's one array: calls that overlap in time on one GPU (two sims' streams,
+ * or two host threads) race on it and must be serialised by the caller.
+ * The array keeps the last call's contents; what must start at zero (the
+ * work counter) is cleared by a kernel on the sim's stream before the step.
+ * Asynchronous on the handle's stream, as arcx_step_device. */
+#define ARCX_NOALLOC_MAX_WORLDS 8192
+int arcx_step_device_noalloc(arc_sim *s, const float *targets_dev);
+
 /* Benchmark helper: `steps` control steps back to back, step k reading
  * targets_dev + k * step_floats (the buffer must hold
  * (steps - 1) * step_floats + N * 12 floats; step_floats = 0 repeats one set);
