@@ -219,6 +219,7 @@ hipError_t prepare_launch(arc_sim *s)
 hipError_t launch_lane_all(arc_sim *s, const float *targets)
 {
     hipError_t e;
+    note_contacts(s);
     if (is_leg(s) && (e = ensure_leg(s)) != hipSuccess)
         return e;
     if (!s->persistent)
@@ -299,6 +300,7 @@ int arcx_step_device_noalloc(arc_sim *s, const float *targets_dev)
     HIPCHK_RET(s, launch_step(d, s->d_model, targets_dev, nullptr, s->n, s->block, tile_of(s), grid, work, is_leg(s),
                               s->stream),
                "arcx_step_device_noalloc", -1);
+    s->contact_layout = 0; /* the lists are in the shared static array, which later calls overwrite */
     return 0;
 }
 
@@ -417,6 +419,7 @@ static int step_batch(arc_sim *s, const int *starts, const int *counts, int n_re
         b->last_bytes = bytes;
     }
     HIPCHK_RET(s, hipEventRecord(b->ev[slot], s->stream), "arcx_step_batch", -1);
+    note_contacts(s);
     if (!graph) {
         if (total == 0 && !task)
             return 0;

@@ -37,6 +37,8 @@ struct DevWorlds {
     float *target;    /* [NJ][stride] drive targets (clamped) */
     float *jout;      /* [3*NJ][stride] joint angle, rate, torque */
     uint8_t *touch;   /* [NS][stride] substeps in contact during the last step */
+    uint8_t *ncontact; /* [NCB][stride] contacts of each contact body recorded in the last substep of the
+                          last step (0 after a reset), the records themselves in cbuf or lcbuf */
     uint32_t *anchored; /* [stride] friction-anchor bit per sphere */
     float *anc;       /* [NS*3][stride] friction anchor */
     float *ancn;      /* [NS*3][stride] anchor normal */
@@ -197,5 +199,8 @@ hipError_t launch_terrain_at(const DevWorlds &d, const Model *m_dev, const int *
                              double *out, hipStream_t s);
 /* full state of world i as floats (layout of the oracle's state dump) */
 hipError_t launch_dump(const DevWorlds &d, int i, float *out, hipStream_t s);
+/* the last substep's contacts of world i (k_dump_contacts): out[0] count,
+ * then 7 floats per contact (body, local point, normal), at most 128 */
+hipError_t launch_dump_contacts(const DevWorlds &d, const Model *m_dev, int i, int layout, float *out, hipStream_t s);
 
 } /* namespace xpbd */

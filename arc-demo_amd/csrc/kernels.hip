@@ -22,7 +22,7 @@ static size_t tcells_of(int grid) { return (size_t)(grid + 2 * TPAD) * (grid + 2
 size_t dev_bytes_per_world(int grid, int cslots)
 {
     return sizeof(float) * ((size_t)SOA_FLOATS + 2 + (size_t)cslots * 6) + tcells_of(grid) + NS /* touch */
-           + sizeof(uint32_t) + 2 * sizeof(int);
+           + NCB /* contact counts */ + sizeof(uint32_t) + 2 * sizeof(int);
 }
 
 hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
@@ -34,7 +34,8 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
     d.tcells = (int)tcells_of(grid);
     size_t S = d.stride;
     size_t floats = S * ((size_t)SOA_FLOATS + 2 + (size_t)cslots * 6);
-    size_t bytes = floats * sizeof(float) + S * (size_t)d.tcells + S * NS + S * sizeof(uint32_t) + 2 * S * sizeof(int) + 1024;
+    size_t bytes = floats * sizeof(float) + S * (size_t)d.tcells + S * NS + S * NCB + S * sizeof(uint32_t)
+                   + 2 * S * sizeof(int) + 1024;
     char *p = nullptr;
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e != hipSuccess)
@@ -61,7 +62,8 @@ hipError_t dev_alloc(DevWorlds &d, int n, int grid, int cslots)
     d.err = d.unstable + S;
     d.anchored = (uint32_t *)(d.err + S);
     d.touch = (uint8_t *)(d.anchored + S);
-    d.tq = (int8_t *)(d.touch + S * NS);
+    d.ncontact = d.touch + S * NS;
+    d.tq = (int8_t *)(d.ncontact + S * NCB);
     return hipSuccess;
 }
 
@@ -229,6 +231,8 @@ __global__ __launch_bounds__(RESET_LANES) void k_reset(DevWorlds d, const Model 
         else
             d.ancn[(sp * 3 + f - 4) * S + i] = 0.0f;
     }
+    if (tid < NCB)
+        d.ncontact[(size_t)tid * S + i] = 0; /* no contacts after a reset (csrc/arc_phys.c:780) */
     if (tid == 0) {
         d.anchored[i] = 0;
         d.unstable[i] = 0;
@@ -350,6 +354,8 @@ __global__ void k_init_template(DevWorlds d, const TemplateK *__restrict__ tk, f
                 d.ancn[(s * 3 + c) * S + i] = 0.0f;
             }
         }
+        for (int c = 0; c < NCB; ++c)
+            d.ncontact[(size_t)c * S + i] = 0;
         d.anchored[i] = 0;
         d.unstable[i] = 0;
         d.time[i] = 0.0f;
@@ -516,6 +522,57 @@ __global__ void k_dump(DevWorlds d, int i, float *out)
     }
     out[o++] = (float)d.unstable[i];
     out[o++] = d.time[i];
+}
+
+/* the contacts world i recorded in the last substep of its last step, in
+ * the reference's record order (spheres in index order: the torso's, then
+ * each leg's femur sphere and tibia spheres, csrc/arc_phys.c:488-500):
+ * out[0] = count, then per contact body index, local point (3), normal (3).
+ * layout: where the step left its lists (1: cbuf, slots of the contact
+ * body's range; 2 / 3: lcbuf, four / two lanes per world, every lane's list
+ * holding the torso's and its own legs' contacts). */
+__global__ void k_dump_contacts(DevWorlds d, const Model *__restrict__ m, int i, int layout, float *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const Model &M = *m;
+    const size_t S = d.stride;
+    int o = 1, total = 0;
+    for (int e = 0; e < NCB; ++e) {
+        /* torso, then leg g's femur (contact body 1 + g) and tibia (5 + g) */
+        const int g = (e - 1) / 2;
+        const int ci = e == 0 ? 0 : ((e - 1) % 2 == 0 ? 1 + g : 5 + g);
+        const int kind = ci == 0 ? 0 : ci <= LEGS ? 1 : 2;
+        const int n = layout ? d.ncontact[(size_t)ci * S + i] : 0;
+        const float *base;
+        size_t stride, w;
+        int slot0;
+        if (layout == 1) {
+            base = d.cbuf;
+            stride = S;
+            w = (size_t)i;
+            slot0 = M.cslot[ci];
+        } else {
+            const int K = layout == 2 ? 1 : 2, P = LEGS / K, q = kind == 0 ? 0 : g / K, l = kind == 0 ? 0 : g % K;
+            base = d.lcbuf;
+            stride = (size_t)P * S;
+            w = (size_t)i * P + q;
+            slot0 = kind == 0 ? 0 : kind == 1 ? M.ccap[0] + l * M.ccap[1] : M.ccap[0] + K * M.ccap[1] + l * M.ccap[5];
+        }
+        for (int k = 0; k < n && total < MAXC; ++k, ++total) {
+            const float *p = base + (size_t)(slot0 + k) * 6 * stride + w;
+            out[o++] = (float)cbody(ci);
+            for (int f = 0; f < 6; ++f)
+                out[o++] = p[f * stride];
+        }
+    }
+    out[0] = (float)total;
+}
+
+hipError_t launch_dump_contacts(const DevWorlds &d, const Model *m_dev, int i, int layout, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dump_contacts, dim3(1), dim3(64), 0, s, d, m_dev, i, layout, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_dump(const DevWorlds &d, int i, float *out, hipStream_t s)

@@ -127,6 +127,15 @@ struct ContactStore {
     float *base;   /* [slot * 6 + field] * stride + world */
     size_t stride;
     size_t world;
+    /* contact counts of the last substep, [contact body][cstride] + cworld
+     * (null: not kept) */
+    uint8_t *counts = nullptr;
+    size_t cstride = 0, cworld = 0;
+    XD void count(int ci, int n) const
+    {
+        if (counts)
+            counts[(size_t)ci * cstride + cworld] = (uint8_t)n;
+    }
     XD void put(int slot, V3 l, V3 n) const
     {
         float *p = base + (size_t)slot * 6 * stride + world;
@@ -225,6 +234,8 @@ XD int lane_step(LaneWorld &W, Anchors &A0, Poses &P0, const Model &M, const Ter
                 cs.get(M.cslot[ci] + k, l, nn);
                 contact_velocity<body_kind(B)>(W.b[B], robot::body[B], l, nn);
             }
+            if (sub == M.substeps - 1)
+                cs.count(ci, n); /* the last substep's records, for the host snapshot */
         });
     }
 

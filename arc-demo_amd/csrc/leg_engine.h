@@ -412,6 +412,11 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
                 else
                     contact_velocity<KANI>(W.b[l][KIND], robot::body[1 + KIND], lp, nn);
             }
+            /* the last substep's records, for the host snapshot: contact body
+             * 0 the torso (lane 0 of the group), 1 + g leg g's femur, 5 + g its
+             * tibia */
+            if (sub0 == M.substeps - 1 && (KIND != 0 || q0 == 0))
+                cs.count(KIND == 0 ? 0 : KIND == 1 ? 1 + q0 * K + l : 5 + q0 * K + l, n);
         });
     }
 

@@ -16,6 +16,7 @@
 #include "reset_engine.h"
 #include "noalloc.h"
 #include "sim_internal.h"
+#include "visuals.h"
 
 void set_error(const arc_sim *cs, int code, const char *what, hipError_t e)
 {
@@ -104,6 +105,7 @@ bool device_memory(const arc_sim *s, const void *p, size_t bytes)
 /* the configured form of the step kernel over all worlds */
 hipError_t launch_form(arc_sim *s, const float *tf)
 {
+    note_contacts(s);
     if (s->form == 1)
         return launch_step_wave(s->dev, s->d_model, s->d_wave, nullptr, tf, 0, s->stream);
     return launch_lane_all(s, tf);
@@ -715,8 +717,25 @@ const arc_world *arc_sim_world(const arc_sim *cs, int i)
     w->n_bodies = T.nb;
     w->n_joints = T.nj;
     w->n_spheres = T.ns;
-    w->n_contacts = 0;
-    w->n_visuals = 0;
+    w->n_visuals = robot_visuals(w->visuals, ARC_MAX_VISUALS);
+    /* the contacts of the last substep of the world's last step, in the
+     * reference's record order */
+    {
+        std::vector<float> c(1 + 7 * MAXC);
+        HIPCHK_RET(s, launch_dump_contacts(s->dev, s->d_model, i, s->contact_layout, s->d_dump, s->stream),
+                   "arc_sim_world", nullptr);
+        HIPCHK_RET(s, hipMemcpyAsync(c.data(), s->d_dump, c.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream),
+                   "arc_sim_world", nullptr);
+        HIPCHK_RET(s, hipStreamSynchronize(s->stream), "arc_sim_world", nullptr);
+        w->n_contacts = (int)c[0];
+        for (int k = 0; k < w->n_contacts; ++k) {
+            const float *r = &c[1 + 7 * k];
+            arc_contact &C = w->contacts[k];
+            C.body = (int)r[0];
+            C.local.x = r[1]; C.local.y = r[2]; C.local.z = r[3];
+            C.n.x = r[4]; C.n.y = r[5]; C.n.z = r[6];
+        }
+    }
     int o = 0;
     for (int b = 0; b < NB; ++b) {
         arc_body &B = w->bodies[b];

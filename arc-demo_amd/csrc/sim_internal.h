@@ -51,7 +51,13 @@ struct arc_sim {
     int err;
     char msg[256];
     BatchState *batch;     /* batched / graph form (batch.cpp), created on first use */
+    int contact_layout;    /* where the last step left its contact lists: 0 nowhere readable (the
+                              allocation-free step), 1 cbuf (one world per lane, plain form), 2 lcbuf with
+                              four lanes per world, 3 lcbuf with two */
 };
+
+/* records the contact-list layout of the configured form, before its step is enqueued */
+inline void note_contacts(arc_sim *s) { s->contact_layout = s->form <= 1 ? 1 : s->form == 3 ? 3 : 2; }
 
 /* records the first error of a handle (sticky) and prints it */
 void set_error(const arc_sim *cs, int code, const char *what, hipError_t e);

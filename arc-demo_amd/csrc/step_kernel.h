@@ -199,6 +199,9 @@ __device__ __forceinline__ void step_world(const DevWorlds &d, const Model &M, c
     cs.base = d.cbuf;
     cs.stride = S;
     cs.world = i;
+    cs.counts = d.ncontact;
+    cs.cstride = S;
+    cs.cworld = i;
     int ov;
     if constexpr (TILE > 0) {
         const LdsTile<BLOCK, TILE> tile =
@@ -346,6 +349,9 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
     cs.base = d.lcbuf;
     cs.stride = P * S;
     cs.world = (size_t)i * P + q;
+    cs.counts = d.ncontact;
+    cs.cstride = S;
+    cs.cworld = (size_t)i;
     constexpr int WPB = BLOCK / P;
     int ov;
     if constexpr (TILE > 0) {
@@ -436,6 +442,9 @@ __device__ __forceinline__ void step_world_shared(const DevWorlds &d, const Mode
     cs.base = live ? d.lcbuf : d.lcdummy;
     cs.stride = live ? 4 * S : (size_t)256;
     cs.world = live ? (size_t)i * 4 + q : (size_t)threadIdx.x;
+    cs.counts = live ? d.ncontact : nullptr; /* lanes without a world of their own store nothing */
+    cs.cstride = S;
+    cs.cworld = (size_t)i;
     const LdsTile<WPB, 16> tile = fill_tile<WPB, 16>(s_tile + threadIdx.x / 4, d, M.terr, (size_t)i, d.x[0 * S + i],
                                                      d.x[1 * S + i]);
     load_leg<1>(d, i, q, W);

@@ -273,6 +273,7 @@ static hipError_t enqueue_step(arc_task *t, const float *actions)
 int arct_step(arc_task *t, const float *actions_dev, int graph)
 {
     arc_sim *s = t->sim;
+    note_contacts(s);
     const float *a = actions_dev ? actions_dev : t->actions;
     if (actions_dev && (!graph || !t->exec || t->graph_actions != a) && !device_memory(s, actions_dev, (size_t)s->n * NJ * sizeof(float))) {
         set_error(s, -1, "arct_step: actions are not device memory of the sim's GPU", hipSuccess);

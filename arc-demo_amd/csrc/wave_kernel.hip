@@ -244,8 +244,17 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
         }
         unsigned long long an = __ballot(lane < T.ns && W.anchored[lane]);
         int ov = 0;
-        if (lane < T.ncb)
+        if (lane < T.ncb) {
             ov = W.ccount[T.cb_body[lane]] > T.cb_cap[lane] ? (1 << lane) : 0;
+            /* the last substep's contacts, for the host snapshot: copied from
+             * LDS into the one-world-per-lane form's list layout */
+            const int B = T.cb_body[lane], ci = cbody_index(B);
+            const int n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
+            for (int k = 0; k < n; ++k)
+                for (int f = 0; f < 6; ++f)
+                    d.cbuf[((size_t)(M.cslot[ci] + k) * 6 + f) * S + i] = W.cl[T.cb_slot[lane] + k][f];
+            d.ncontact[(size_t)ci * S + i] = (uint8_t)n;
+        }
         for (int off = 32; off > 0; off >>= 1)
             ov |= __shfl_xor(ov, off);
         if (lane == 0) {

@@ -73,6 +73,10 @@ typedef struct {
     vec3 n;
 } arc_contact;
 
+/* visual shapes and surfaces (csrc/arc_phys.h:69-87) */
+enum { ARC_VIS_SPHERE, ARC_VIS_BOX, ARC_VIS_TAPER, ARC_VIS_CYLINDER, ARC_VIS_RING };
+enum { ARC_MAT_PAINT, ARC_MAT_PANELS, ARC_MAT_METAL, ARC_MAT_RUBBER, ARC_MAT_LENS, ARC_MAT_RIBS, ARC_MAT_GLOW };
+
 typedef struct {
     int type;
     int body;

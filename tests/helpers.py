@@ -87,6 +87,108 @@ def assert_rel(a, b, rel: float = 1e-5, what: str = "") -> None:
     assert ok.all(), f"{what}: {_first_diff(ok, a, b)}"
 
 
+def world_type(real):
+    """ctypes mirror of arc_world (include/arc_world.h, the reference's
+    csrc/arc_phys.h:25-112 layout) with `real` for every double: c_double
+    for the library's snapshot and the FP64 reference build, c_float for the
+    reference compiled with double as float."""
+    C = ctypes
+    f = lambda *names: [(n, real) for n in names]
+
+    class V3(C.Structure):
+        _fields_ = f("x", "y", "z")
+
+    class Q(C.Structure):
+        _fields_ = f("w", "x", "y", "z")
+
+    class M3(C.Structure):
+        _fields_ = [("m", real * 9)]
+
+    class Body(C.Structure):
+        _fields_ = [("x", V3), ("x_prev", V3), ("q", Q), ("q_prev", Q), ("v", V3), ("w", V3)] + \
+                   f("mass", "inv_mass") + [("inertia", V3), ("inv_inertia", V3), ("rot", M3), ("inv_iw", M3)]
+
+    class Sphere(C.Structure):
+        _fields_ = [("body", C.c_int), ("pos", V3), ("radius", real), ("touch", C.c_int), ("anchored", C.c_int),
+                    ("anchor", V3), ("anchor_n", V3)]
+
+    class Joint(C.Structure):
+        _fields_ = [("parent", C.c_int), ("child", C.c_int), ("r1", V3), ("r2", V3), ("a1", V3), ("a2", V3),
+                    ("b1", V3), ("b2", V3), ("q_rel0", Q)] + \
+                   f("lo", "hi", "kp", "kd", "max_torque", "target", "angle", "rate", "torque", "torque_acc") + \
+                   [("lim", real * 4)]
+
+    class Contact(C.Structure):
+        _fields_ = [("body", C.c_int), ("local", V3), ("n", V3)]
+
+    class Visual(C.Structure):
+        _fields_ = [("type", C.c_int), ("body", C.c_int), ("p0", V3), ("p1", V3)] + \
+                   f("radius", "radius2", "aspect") + [("material", C.c_int), ("color", C.c_float * 3)]
+
+    class Terrain(C.Structure):
+        _fields_ = [("nx", C.c_int), ("ny", C.c_int)] + f("cell", "ox", "oy", "outside") + [("h", real * 4096)]
+
+    class World(C.Structure):
+        _fields_ = [("gravity", V3)] + f("mu_static", "mu_dynamic") + \
+                   [(n, C.c_int) for n in ("n_bodies", "n_joints", "n_spheres", "n_contacts", "n_visuals",
+                                           "unstable")] + \
+                   [("time", real), ("bodies", Body * 16), ("joints", Joint * 16), ("spheres", Sphere * 48),
+                    ("contacts", Contact * 128), ("visuals", Visual * 80), ("terrain", Terrain)]
+
+    return World
+
+
+def struct_values(obj, skip=()) -> dict:
+    """Flattens a ctypes structure into {dotted name: float or int}."""
+    out = {}
+
+    def walk(o, prefix):
+        if isinstance(o, ctypes.Structure):
+            for name, _ in o._fields_:
+                if name not in skip:
+                    walk(getattr(o, name), f"{prefix}{name}.")
+        elif isinstance(o, ctypes.Array):
+            for k in range(len(o)):
+                walk(o[k], f"{prefix}{k}.")
+        else:
+            out[prefix[:-1]] = o
+
+    walk(obj, "")
+    return out
+
+
+class TaskMirror:
+    """The device's fall rules and auto-reset replayed on an oracle sim
+    (arc/task.py:196-203 for the causes, arc/env.py:114-116 for the reset of
+    the ended worlds): after each step every fallen world is reset with seed
+    global_index + n_seed * episode (its episode count after this fall) and
+    its own difficulty, and its torso-contact counter restarts, as the
+    device's k_falls / k_reset do.  `index` maps the oracle's worlds to their
+    indices in the device sim (a sample, or a client's range), `n_seed` is
+    the device sim's world count."""
+
+    def __init__(self, oracle_sim, index, n_seed, difficulty, beached=30, flip=0.3, void=-2.0):
+        self.o = oracle_sim
+        self.index = np.asarray(index, np.int64)
+        self.n_seed = int(n_seed)
+        self.diff = np.asarray(difficulty, np.float64)
+        self.rules = (beached, flip, void)
+        self.torso = np.zeros(len(self.index), np.int32)
+        self.episode = np.zeros(len(self.index), np.int64)
+        self.falls = 0
+
+    def after_step(self) -> np.ndarray:
+        cause = self.o.falls(self.torso, *self.rules)
+        fell = np.flatnonzero(cause)
+        if len(fell):
+            self.episode[fell] += 1
+            seeds = ((self.index[fell] + self.n_seed * self.episode[fell]) & 0xFFFFFFFF).astype(np.uint32)
+            self.o.reset(fell.astype(np.int32), seeds, self.diff[fell])
+            self.torso[fell] = 0
+            self.falls += len(fell)
+        return cause
+
+
 class HostCheck:
     """The device engine's code compiled for the host (tests/hostcheck)."""
 

@@ -151,22 +151,6 @@ def test_heights_and_terrain_at(native):
     assert np.array_equal(g.terrain_at(ids, xy), o.terrain_at(ids, xy))
 
 
-def test_world_mirror(native):
-    g = native.Sim(3)
-    g.reset(np.arange(3), np.arange(3), np.full(3, 0.5))
-    g.step(action_stream(3, 1)[0])
-    import ctypes
-    lib = native.sim_lib()
-    p = lib.arc_sim_world(g.ptr, 1)
-    assert p
-    st = g.state(1)
-    # arc_world: gravity(24) mu(16) 5 counts + unstable (24) time(8) then bodies
-    raw = (ctypes.c_double * 3).from_address(p + 24 + 16 + 24 + 8)
-    assert raw[0] == st[0] and raw[1] == st[1] and raw[2] == st[2]
-    counts = (ctypes.c_int * 6).from_address(p + 40)
-    assert list(counts)[:3] == [13, 12, 21]
-
-
 @pytest.mark.parametrize("difficulty", [0.0, "mixed"])
 def test_wave_form_matches_lane_form(native, difficulty):
     """The plain form (one world per wavefront, joints by dependency level,
