@@ -77,6 +77,8 @@ def actions_to_targets(defaults: np.ndarray, u: np.ndarray) -> np.ndarray:
 
 MAX_SETS = 512
 TUNE_STEPS = 8   # timed steps per candidate in the autotuner (fewer let run-to-run noise pick a slower shape)
+SETTLE = 100     # steps (fall rules and auto-reset on) every pass takes before --warmup and the timed ones:
+                 # the contact load of freshly reset worlds settles after ~100 steps
 
 
 def target_pool(defaults: np.ndarray, n: int, sets: int, seed: int = 0) -> np.ndarray:
@@ -109,31 +111,55 @@ def new_sim(n: int, difficulty=None):
     return s
 
 
-def warm_and_tune(s, n, t, warm, sets):
-    """--warmup steps with resets (the contact load settles after ~100 steps),
-    then the autotuner on the settled worlds; t holds `sets` target sets"""
+def settle(s, n, t, warm, sets):
+    """SETTLE + --warmup steps with the fall rules and auto-reset on, target
+    set k at step k; returns the steps taken"""
     per = n * NJ
-    for k in range(warm):
+    for k in range(SETTLE + warm):
         s.step_batch([0], [n], t.ptr + 4 * per * (k % sets), task=True, graph=True)
     s.sync()
+    return SETTLE + warm
+
+
+def warm_and_tune(s, n, t, warm, sets):
+    """settle, then the autotuner on the settled worlds; t holds `sets`
+    target sets.  Returns (launch, steps taken)."""
+    k = settle(s, n, t, warm, sets)
     cfg, _ = s.autotune(TUNE_STEPS)
-    return cfg
+    return cfg, k
 
 
-def kernel_pass(steps: int, warm: int) -> dict:
-    """the step kernel at 65,536 worlds: HIP events around each launch on the engine stream"""
+def apply_launch(s, launch: dict) -> None:
+    """puts a sim in a launch configuration reported by launch()/autotune()"""
+    s.set_form(launch["form"])
+    if launch["form"] != "wave":
+        s.set_launch(int(launch["block"]), 0 if launch["persistent"] else int(launch["grid_blocks"]))
+        s.set_layout(int(launch["tile"]))
+        s.set_persistent(bool(launch["persistent"]))
+
+
+# the launch shape of a new sim (four lanes per world, 64-thread workgroups)
+DEFAULT_LAUNCH = {"block": 64, "tile": 16, "grid_blocks": 0, "persistent": 0, "form": "leg"}
+
+
+def kernel_pass(steps: int, warm: int, launch: dict | None = None) -> dict:
+    """the step kernel at 65,536 worlds in the given launch configuration
+    (the served headline's; a new sim's by default), from settled worlds: HIP
+    events around each launch on the engine's stream"""
     from arc import native
     s = new_sim(N)
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, S))
     per = N * NJ
-    cfg = warm_and_tune(s, N, t, warm, S)
+    k0 = settle(s, N, t, warm, S)
+    apply_launch(s, launch or DEFAULT_LAUNCH)
     kms = 0.0
     for k in range(steps):
-        m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
+        m, _ = s.time_steps(t.ptr + 4 * per * ((k0 + k) % S), 1, per)
         kms += m
     avg_s = kms / 1e3 / steps
-    return {"launch": cfg, "kernel_ms_avg": 1e3 * avg_s, "launches": steps}
+    return {"launch": s.launch(), "kernel_ms_avg": 1e3 * avg_s, "launches": steps,
+            "settled_steps": k0}
 
 
 def forms_pass(steps: int, warm: int) -> dict:
@@ -143,7 +169,7 @@ def forms_pass(steps: int, warm: int) -> dict:
     from arc import native
     shapes = {"lane": 64, "leg": 64, "pair": 64, "shared": 256, "wave": 64}
     out = {}
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     defaults = native.Sim(1).default_angles
     for n in (N, 4096):
         t = native.DeviceBuffer.from_array(target_pool(defaults, n, S))
@@ -152,15 +178,13 @@ def forms_pass(steps: int, warm: int) -> dict:
         for form, block in shapes.items():
             s = new_sim(n)
             s.set_form("leg")
-            for k in range(warm):
-                s.step_batch([0], [n], t.ptr + 4 * per * (k % S), task=True, graph=True)
-            s.sync()
+            k0 = settle(s, n, t, warm, S)
             s.set_form(form)
             s.set_launch(block, 0)
             s.set_layout(16)
             kms = 0.0
             for k in range(steps):
-                m, _ = s.time_steps(t.ptr + 4 * per * ((warm + k) % S), 1, per)
+                m, _ = s.time_steps(t.ptr + 4 * per * ((k0 + k) % S), 1, per)
                 kms += m
             res[form] = {"launch": s.launch(), "kernel_ms_avg": kms / steps}
             s.close()
@@ -175,19 +199,19 @@ def forms_pass(steps: int, warm: int) -> dict:
 def action_law_pass(steps: int, warm: int) -> dict:
     """the step kernel at 65,536 worlds under action laws that repeat with
     period p (p sets of the law in turn) and with fresh actions at every step:
-    a new sim per law, --warmup steps of that law with resets, then --steps
-    launches timed with HIP events"""
+    a new sim per law, SETTLE + --warmup steps of that law with resets, then
+    --steps launches timed with HIP events"""
     from arc import native
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     t = native.DeviceBuffer.from_array(target_pool(native.Sim(1).default_angles, N, S))
     per = N * NJ
     out = {}
     for p in (2, 4, 8, 32, S):
         s = new_sim(N)
-        for k in range(warm):
+        for k in range(SETTLE + warm):
             s.step_batch([0], [N], t.ptr + 4 * per * (k % p), task=True, graph=True)
         s.sync()
-        kms = sum(s.time_steps(t.ptr + 4 * per * ((warm + k) % p), 1, per)[0] for k in range(steps))
+        kms = sum(s.time_steps(t.ptr + 4 * per * ((SETTLE + warm + k) % p), 1, per)[0] for k in range(steps))
         out["fresh" if p == S else f"period_{p}"] = {"period": p, "kernel_ms_avg": kms / steps,
                                                      "launch": s.launch()}
         s.close()
@@ -198,13 +222,13 @@ def action_law_pass(steps: int, warm: int) -> dict:
 def batched_pass(steps: int, warm: int) -> dict:
     from arc import native
     s = new_sim(N)
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, S))
     per = N * NJ
-    cfg = warm_and_tune(s, N, t, warm, S)
+    cfg, k0 = warm_and_tune(s, N, t, warm, S)
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step_batch([0], [N], t.ptr + 4 * per * ((warm + k) % S), task=True, graph=True)
+        s.step_batch([0], [N], t.ptr + 4 * per * ((k0 + k) % S), task=True, graph=True)
     s.sync()
     el = time.perf_counter() - t0
     stats = s.task_stats()
@@ -217,10 +241,10 @@ def abi_pass(steps: int, warm: int) -> dict:
     step, return on completion (PCIe-inclusive; never the headline)"""
     from arc import native
     s = new_sim(N)
-    S = pool_sets(warm, 0)
+    S = pool_sets(SETTLE + warm, 0)
     hp = target_pool(s.default_angles, N, S + steps + 1)
     t = native.DeviceBuffer.from_array(hp[:S])
-    cfg = warm_and_tune(s, N, t, warm, S)
+    cfg, _ = warm_and_tune(s, N, t, warm, S)
     host = [hp[S + k].astype(np.float64) for k in range(steps + 1)]
     s.step(host[steps])
     t0 = time.perf_counter()
@@ -231,35 +255,61 @@ def abi_pass(steps: int, warm: int) -> dict:
             "path": "arc_sim_step(host float64 targets): convert, copy to HBM, step, synchronize"}
 
 
-def plain_pass(steps: int = 3) -> dict:
+def plain_pass(warm: int, steps: int = 3) -> dict:
+    """the plain form from the settled state of the other passes (settled in
+    the four-lanes-per-world form, which computes the same bits), one launch
+    per control step, fall rules and auto-reset on"""
     from arc import native
     s = new_sim(N)
-    s.set_form("wave")
-    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, steps + 1))
+    S = pool_sets(SETTLE + warm, steps)
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, N, S))
     per = N * NJ
-    s.step_task(t.ptr)
+    k0 = settle(s, N, t, warm, S)
+    s.set_form("wave")
+    s.step_task(t.ptr + 4 * per * (k0 % S))  # the plain kernel's first launch outside the timed region
     s.sync()
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step_task(t.ptr + 4 * per * (k + 1))
+        s.step_task(t.ptr + 4 * per * ((k0 + 1 + k) % S))
     s.sync()
     el = time.perf_counter() - t0
-    return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+    return {"value": N * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps, "settled_steps": k0 + 1,
             "form": "one world per wavefront, world staged in LDS, one launch per step"}
+
+
+def config1_pass(steps: int = 1000) -> dict:
+    """64 worlds on flat ground (difficulty 0), seeds 0..63, 1000 control
+    steps with the fall rules and auto-reset on the device, one launch
+    sequence per step from device-resident targets, timed end to end"""
+    from arc import native
+    n = 64
+    s = new_sim(n, 0.0)
+    t = native.DeviceBuffer.from_array(target_pool(s.default_angles, n, steps + 1, seed=300))
+    per = n * NJ
+    s.step_task(t.ptr + 4 * per * steps)  # first launches outside the timed region
+    s.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step_task(t.ptr + 4 * per * k)
+    s.sync()
+    el = time.perf_counter() - t0
+    stats = s.task_stats()
+    return {"worlds": n, "steps": steps, "value": n * steps / el, "ms_per_step": 1e3 * el / steps,
+            "resets": int(stats[1:].sum()), "launch": s.launch()}
 
 
 def latency_pass(steps: int, warm: int) -> dict:
     from arc import native
     n = 4096
     s = new_sim(n, 0.5)
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     t = native.DeviceBuffer.from_array(target_pool(s.default_angles, n, S))
     per = n * NJ
-    cfg = warm_and_tune(s, n, t, warm, S)
+    cfg, k0 = warm_and_tune(s, n, t, warm, S)
     lat = []
     for k in range(steps):
         t0 = time.perf_counter()
-        s.step_batch([0], [n], t.ptr + 4 * per * ((warm + k) % S), task=True, graph=True)
+        s.step_batch([0], [n], t.ptr + 4 * per * ((k0 + k) % S), task=True, graph=True)
         s.sync()
         lat.append(1e3 * (time.perf_counter() - t0))
     lat = np.array(lat)
@@ -280,7 +330,7 @@ def served_pass(steps: int, warm: int) -> dict:
     sv = server.StepServer(N * per_world * 1.0001)
     n = N // CLIENTS
     cids = [sv.open(n, seed0=c * n, difficulty=-1.0) for c in range(CLIENTS)]
-    S = pool_sets(warm, steps)
+    S = pool_sets(SETTLE + warm, steps)
     hp = target_pool(native.Sim(1).default_angles, N, S, seed=1000)
     dev = native.DeviceBuffer.from_array(hp)
 
@@ -301,9 +351,9 @@ def served_pass(steps: int, warm: int) -> dict:
                 sv.wait(x)
 
     out = {"clients": CLIENTS, "worlds_per_client": n, "targets": "device-resident (arcs_submit_device)"}
-    run(0, warm, 1, True)
+    run(0, SETTLE + warm, 1, True)
     out["launch"], _ = sv.autotune(TUNE_STEPS)
-    k = warm
+    k = SETTLE + warm
     for depth, device, name in ((1, True, "closed_loop"), (2, True, "pipelined"), (2, False, "pipelined_host_pcie")):
         run(k, warm, depth, device)
         k += warm
@@ -493,7 +543,34 @@ def _ref_lib():
     return P, kind
 
 
-def cpu_end_to_end(n: int, seconds: float) -> dict:
+def host_info() -> dict:
+    """the CPUs this job sees: model, visible logical CPUs, physical cores,
+    the affinity mask, OMP_NUM_THREADS, and the thread count the baselines
+    use (the affinity mask, capped by OMP_NUM_THREADS when it is set: the
+    share of the host's CPUs this job may use)"""
+    import os
+    text = open("/proc/cpuinfo").read()
+    model = next((l.split(":", 1)[1].strip() for l in text.splitlines() if l.startswith("model name")), "?")
+    cores = set()
+    phys = core = None
+    for l in text.splitlines():
+        if l.startswith("physical id"):
+            phys = l.split(":", 1)[1].strip()
+        elif l.startswith("core id"):
+            core = l.split(":", 1)[1].strip()
+        elif not l.strip() and core is not None:
+            cores.add((phys, core))
+            phys = core = None
+    if core is not None:
+        cores.add((phys, core))
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    use = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    return {"cpu_model": model, "visible_cpus": os.cpu_count(), "physical_cores": len(cores) or None,
+            "affinity_cpus": affinity, "omp_num_threads": omp, "threads_used": use}
+
+
+def cpu_end_to_end(n: int, seconds: float, threads: int, max_steps: int | None = None) -> dict:
     """The reference's own step loop with its fall rules and auto-reset
     (arc/task.py:196-203 and arc/env.py:108-116 restated in numpy over the
     reference engine, oracle/_ref, shipped flags): step, observe, falls,
@@ -501,9 +578,8 @@ def cpu_end_to_end(n: int, seconds: float) -> dict:
     P, kind = _ref_lib()
     if kind is None:
         return {"value": None, "unit": "world-steps/s", "note": "oracle/_ref not built"}
-    threads = min(16, P.cpu_threads())
     s = P.Lib(kind).sim(n)
-    d = unit24(rng(1, n))
+    d = unit24(rng(1, n)) if max_steps is None else np.zeros(n)
     ids = np.arange(n, dtype=np.int32)
     s.reset(ids, ids.astype(np.uint32), d)
     defaults = s.robot_info()[0]
@@ -511,7 +587,7 @@ def cpu_end_to_end(n: int, seconds: float) -> dict:
     torso = np.zeros(n, dtype=np.int64)
     episode = np.zeros(n, dtype=np.int64)
     done, k, resets, el = 0, 0, 0, 0.0
-    while el < seconds:
+    while (el < seconds) if max_steps is None else (k < max_steps):
         # fresh actions every step (drawn outside the timed work)
         tg = actions_to_targets(defaults, rng(k, n * NJ)).astype(np.float64).reshape(n, NJ)
         t0 = time.perf_counter()
@@ -530,25 +606,21 @@ def cpu_end_to_end(n: int, seconds: float) -> dict:
         el += time.perf_counter() - t0
         done += n
         k += 1
+    terrain = "flat (difficulty 0)" if max_steps is not None else "difficulty U[0,1)"
     return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"{n} worlds x {k} steps ({el:.1f} s): arc_sim_step, arc_sim_observe, numpy fall rules, "
-                      f"arc_sim_reset of the fallen ({resets} resets); fresh uniform actions every step",
+            "worlds": n, "steps": k,
+            "sample": f"{n} worlds x {k} steps ({el:.1f} s), {terrain}: arc_sim_step, arc_sim_observe, numpy fall "
+                      f"rules, arc_sim_reset of the fallen ({resets} resets); fresh uniform actions every step",
             "build": kind}
 
 
-def cpu_baseline(n: int, seconds: float) -> dict:
-    """Reference engine (oracle/_ref, shipped flags) on this host's cores."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle as P
-    flags = open("/proc/cpuinfo").read()
-    kind = "r64_v4" if all(f in flags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd")) else "r64_v3"
-    if not P.available(kind):
-        kind = "r64_v3" if P.available("r64_v3") else None
+def cpu_baseline(n: int, seconds: float, threads: int) -> dict:
+    """Reference engine (oracle/_ref, shipped flags) on `threads` of this
+    host's cores: arc_sim_step only, timed as csrc/tools/bench.c:40-48 does."""
+    P, kind = _ref_lib()
     if kind is None:
         return {"value": None, "unit": "world-steps/s", "note": "oracle/_ref not built"}
-    threads = min(16, P.cpu_threads())
-    L = P.Lib(kind)
-    s = L.sim(n)
+    s = P.Lib(kind).sim(n)
     d = unit24(rng(1, n))
     ids = np.arange(n, dtype=np.int32)
     s.reset(ids, ids.astype(np.uint32), d)
@@ -562,10 +634,26 @@ def cpu_baseline(n: int, seconds: float) -> dict:
         el += time.perf_counter() - t0
         done += n
         k += 1
-    cpu = flags.split("model name")[1].split("\n")[0].strip(": \t") if "model name" in flags else "?"
-    return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference",
+    return {"value": done / el, "unit": "world-steps/s", "cores": threads, "kind": "reference", "worlds": n,
             "sample": f"{n} worlds x {k} steps ({el:.1f} s of stepping), per-world difficulty U[0,1), fresh "
-                      "uniform actions every step (the GPU passes' law), arc_sim_step only", "build": kind, "cpu": cpu}
+                      "uniform actions every step (the GPU passes' law), arc_sim_step only", "build": kind}
+
+
+def cpu_passes(seconds: float) -> dict:
+    """the reference on all the CPU share this job may use and on half of it,
+    at the metric's 65,536 worlds and at 64; config 1's end-to-end loop (64
+    flat worlds, 1000 steps, fall rules and auto-reset) and the 65,536-world
+    one on all of it"""
+    host = host_info()
+    full = host["threads_used"]
+    half = max(1, full // 2)
+    table = []
+    for n, secs in ((N, seconds), (64, seconds / 2)):
+        for th in (full, half):
+            table.append(cpu_baseline(n, secs, th))
+    return {"host": host, "table": table,
+            "end_to_end_65536": cpu_end_to_end(N, seconds / 2, full),
+            "end_to_end_config1_64": cpu_end_to_end(64, 0.0, full, max_steps=1000)}
 
 
 def main() -> int:
@@ -574,11 +662,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--cpu-worlds", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--latency-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "kernel", "rollout", "forms", "actions"], default="all")
+    ap.add_argument("--only", choices=["all", "kernel", "rollout", "forms", "actions", "cpu"], default="all")
+    ap.add_argument("--block", type=int, default=64, help="--only kernel: workgroup size of the leg form")
     ap.add_argument("--no-rollout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -607,8 +695,11 @@ def main() -> int:
         native.sim_lib().arcx_memcpy(b.ptr, a.ptr, 1 << 30, 2)
         a.close()
         b.close()
-        kp = kernel_pass(steps, warm)
+        kp = kernel_pass(steps, warm, dict(DEFAULT_LAUNCH, block=args.block))
         print(json.dumps({"kernel_pass": kp}), flush=True)
+        return 0
+    if args.only == "cpu":
+        print(json.dumps({"cpu": cpu_passes(args.cpu_seconds)}), flush=True)
         return 0
 
     log("served form, 65536 worlds")
@@ -619,10 +710,12 @@ def main() -> int:
     log(f"batched {batched['value']:.4g}; reference ABI path")
     abi = abi_pass(10, warm)
     log(f"abi {abi['value']:.4g}; kernel pass")
-    kp = kernel_pass(steps, warm)
+    kp = kernel_pass(steps, warm, served["launch"])
     log(f"{kernel_name(kp['launch'])} {kp['kernel_ms_avg']:.3f} ms; plain form")
-    plain = plain_pass()
-    log(f"plain {plain['value']:.4g}; latency at 4096 worlds")
+    plain = plain_pass(warm)
+    log(f"plain {plain['value']:.4g}; 64 flat worlds x 1000 steps")
+    config1 = config1_pass()
+    log(f"config 1 {config1['value']:.4g}; latency at 4096 worlds")
     lat = latency_pass(args.latency_steps, warm)
     log(f"latency p50 {lat['p50_ms']:.3f} p99 {lat['p99_ms']:.3f} ms; serving")
     serving = serving_pass(args.serve_seconds)
@@ -661,21 +754,25 @@ def main() -> int:
                      "flop_per_world_step": FLOP_PER_WORLD_STEP,
                      "hbm_frac": N * BYTES_PER_WORLD_STEP / avg_s / PEAK_HBM},
         "forms": {"served": served, "batched_graph": batched, "plain": plain, "abi_pcie_inclusive": abi},
+        "config1_64_flat_1000_steps": config1,
         "latency_4096": lat,
         "serving": serving,
         "rollout_8192": rollout,
         "launch": kp["launch"],
     }
+    line["ratios"] = {"served_vs_plain": served["value"] / plain["value"],
+                      "batched_vs_plain": batched["value"] / plain["value"]}
     if not args.no_cpu:
-        log("cpu baseline")
-        cpu = cpu_baseline(args.cpu_worlds, args.cpu_seconds)
-        line["cpu_baseline"] = cpu
-        log("cpu end-to-end (falls and resets)")
-        line["cpu_end_to_end"] = cpu_end_to_end(args.cpu_worlds, min(args.cpu_seconds, 5.0))
-        if cpu.get("value"):
-            line["ratios"] = {"served_vs_cpu": served["value"] / cpu["value"],
-                              "served_vs_plain": served["value"] / plain["value"],
-                              "batched_vs_plain": batched["value"] / plain["value"]}
+        log("cpu baselines")
+        cpu = cpu_passes(args.cpu_seconds)
+        head = cpu["table"][0]  # 65,536 worlds on the whole CPU share
+        line["cpu_baseline"] = dict(head, host=cpu["host"])
+        line["cpu"] = cpu
+        if head.get("value"):
+            line["ratios"]["served_vs_cpu"] = served["value"] / head["value"]
+            e2e = cpu["end_to_end_config1_64"]
+            if e2e.get("value"):
+                line["ratios"]["config1_gpu_vs_cpu_end_to_end"] = config1["value"] / e2e["value"]
     print(json.dumps(line), flush=True)
     return 0
 
