@@ -47,6 +47,9 @@ CFG = dict(max_steps=17, checkpoint_radius=4.5, fov=170.0, border=9.0, beached_s
 
 
 def load_reference():
+    # the reference tree is read-only input: importing from it must leave no
+    # __pycache__ behind
+    sys.dont_write_bytecode = True
     pkg = types.ModuleType("refarc")
     pkg.__path__ = [str(REF)]
     sys.modules["refarc"] = pkg
