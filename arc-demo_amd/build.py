@@ -35,7 +35,7 @@ STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
 HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip",
                "noalloc.hip"]
 HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "task.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp",
-                "visuals.cpp"]
+                "visuals.cpp", "world.cpp"]
 
 
 def hipcc() -> str:

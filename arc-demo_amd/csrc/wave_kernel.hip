@@ -13,7 +13,9 @@
  *     serial order's results bit for bit;
  *   - sphere contacts: one lane per body, each walking that body's spheres in
  *     order (a contact solve moves only the sphere's own body,
- *     csrc/arc_phys.c:488-500);
+ *     csrc/arc_phys.c:488-500); where the spheres' cell windows could exceed
+ *     the reference's 128-record cap, one lane walks every sphere in the
+ *     world's order into a single list, so the cap cuts where it does there;
  *   - contact velocities: one lane per body, its contacts in record order.
  * The topology comes from tables built on the host from the world itself, and
  * every body is the run-time-kinded KGEN, so the code follows the reference's
@@ -21,6 +23,7 @@
  */
 #include "device_state.h"
 #include "wave_tables.h"
+#include "world_step.h"
 
 namespace xpbd {
 
@@ -32,12 +35,14 @@ struct WaveWorld {
     V3 anc[WAVE_MAX_SPHERES], ancn[WAVE_MAX_SPHERES];
     Pose0 p0[WAVE_MAX_BODIES];
     float cl[WAVE_MAX_SLOTS][6];
+    int cbody[WAVE_MAX_SLOTS]; /* serial collision: body of each record of the one list */
     int ccount[WAVE_MAX_BODIES];
+    int ntotal;                /* serial collision: records in the list */
     int unstable;
 };
 
-template <int PK>
-__device__ __forceinline__ void wave_joint_pos(WaveWorld &W, const Model &M, const WaveK &T, int j)
+template <int PK, class MT>
+__device__ __forceinline__ void wave_joint_pos(WaveWorld &W, const MT &M, const WaveK &T, int j)
 {
     int p = T.jp[j], c = T.jc[j];
     Body C = W.b[c];
@@ -51,8 +56,8 @@ __device__ __forceinline__ void wave_joint_pos(WaveWorld &W, const Model &M, con
     W.b[c] = C;
 }
 
-template <int PK>
-__device__ __forceinline__ void wave_joint_vel(WaveWorld &W, const Model &M, const WaveK &T, int j)
+template <int PK, class MT>
+__device__ __forceinline__ void wave_joint_vel(WaveWorld &W, const MT &M, const WaveK &T, int j)
 {
     int p = T.jp[j], c = T.jc[j];
     Body C = W.b[c];
@@ -66,21 +71,27 @@ __device__ __forceinline__ void wave_joint_vel(WaveWorld &W, const Model &M, con
     W.b[c] = C;
 }
 
-/* spheres of contact-body group g, in sphere order, against the terrain */
-__device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M, const WaveK &T, int g,
-                                                   const TerrainRef &tr, int &total)
+/* spheres sph[0 .. count) of contact-body group g, in that order, against the
+ * terrain whose columns `cells` returns.  Records go to the group's own slot
+ * range, or, with `serial`, to the world's one list in record order (at most
+ * MAXC records, the reference's cap) tagged with their body. */
+template <class MT, class Cells>
+__device__ __forceinline__ void wave_collide(WaveWorld &W, const MT &M, const WaveK &T, int g, const int *sph, int count,
+                                             const Cells &cells, int &total, bool serial)
 {
     const int B = T.cb_body[g];
     Body b = W.b[B];
     int &cnt = W.ccount[B];
-    for (int k = T.cb_first[g]; k < T.cb_first[g + 1]; ++k) {
-        const int s = T.cb_sphere[k];
+    for (int k = 0; k < count; ++k) {
+        const int s = sph[k];
         auto rec = [&](V3 l, V3 n) {
             if (total < MAXC) {
-                if (cnt < T.cb_cap[g]) {
-                    float *c = W.cl[T.cb_slot[g] + cnt];
+                float *c = serial ? W.cl[total] : cnt < T.cb_cap[g] ? W.cl[T.cb_slot[g] + cnt] : nullptr;
+                if (c) {
                     c[0] = l.x; c[1] = l.y; c[2] = l.z; c[3] = n.x; c[4] = n.y; c[5] = n.z;
                 }
+                if (serial)
+                    W.cbody[total] = B;
                 cnt++;
                 total++;
             }
@@ -88,9 +99,108 @@ __device__ __forceinline__ void wave_collide_group(WaveWorld &W, const Model &M,
         AnchorRef an{ W.anc[s], W.ancn[s] };
         auto pose0 = [&]() { return W.p0[B]; };
         collide<KGEN, AT_GEN>(b, M.body[B], M.sphere[s].at, M.sphere[s].rad, W.touch[s], W.anchored[s], an, pose0,
-                              M.terr, GridCells{ M.terr, tr }, rec, M.mus, M.mud);
+                              M.terr, cells, rec, M.mus, M.mud);
     }
     W.b[B] = b;
+}
+
+/* The substeps of one world step (csrc/arc_phys.c:727-737) on the world
+ * staged in W, by the 64 lanes of the calling wavefront.  The caller has run
+ * the step's prologue (caches, touch counts and torque sums cleared). */
+template <class MT, class Cells>
+__device__ __forceinline__ void wave_substeps(WaveWorld &W, const MT &M, const WaveK &T, const Cells &cells, int lane)
+{
+    for (int sub = 0; sub < M.substeps; ++sub) {
+        if (lane < T.nb) {
+            Body b = W.b[lane];
+            W.p0[lane] = integrate<KGEN>(b, M.body[lane], M.g, M.h);
+            W.b[lane] = b;
+            W.ccount[lane] = 0;
+        }
+        __syncthreads();
+        for (int L = 0; L < T.nlev; ++L) {
+            int k = T.lev_first[L] + lane;
+            if (k < T.lev_first[L + 1]) {
+                int j = T.lev_joint[k];
+                if (T.jp[j] >= 0)
+                    wave_joint_pos<KGEN>(W, M, T, j);
+                else
+                    wave_joint_pos<-1>(W, M, T, j);
+            }
+            __syncthreads();
+        }
+        if (T.serial_collide) {
+            /* the contact-record cap could bind: every sphere in the world's
+             * sphere order, as the reference records them */
+            if (lane == 0) {
+                int total = 0;
+                for (int s = 0; s < T.ns; ++s)
+                    wave_collide(W, M, T, T.s_group[s], &s, 1, cells, total, true);
+                W.ntotal = total;
+            }
+        } else if (lane < T.ncb) {
+            int total = 0; /* the cap cannot bind: groups are independent */
+            wave_collide(W, M, T, lane, &T.cb_sphere[T.cb_first[lane]], T.cb_first[lane + 1] - T.cb_first[lane], cells,
+                         total, false);
+        }
+        __syncthreads();
+        if (lane < T.nb) {
+            Body b = W.b[lane];
+            velocities(b, W.p0[lane], M.inv_h, M.two_h);
+            W.b[lane] = b;
+        }
+        __syncthreads();
+        for (int L = 0; L < T.nlev; ++L) {
+            int k = T.lev_first[L] + lane;
+            if (k < T.lev_first[L + 1]) {
+                int j = T.lev_joint[k];
+                if (T.jp[j] >= 0)
+                    wave_joint_vel<KGEN>(W, M, T, j);
+                else
+                    wave_joint_vel<-1>(W, M, T, j);
+            }
+            __syncthreads();
+        }
+        if (lane < T.ncb) {
+            const int B = T.cb_body[lane];
+            Body b = W.b[B];
+            if (T.serial_collide) {
+                /* the body's records in record order */
+                for (int k = 0; k < W.ntotal; ++k)
+                    if (W.cbody[k] == B) {
+                        const float *c = W.cl[k];
+                        contact_velocity<KGEN>(b, M.body[B], v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]));
+                    }
+            } else {
+                int n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
+                for (int k = 0; k < n; ++k) {
+                    const float *c = W.cl[T.cb_slot[lane] + k];
+                    contact_velocity<KGEN>(b, M.body[B], v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]));
+                }
+            }
+            W.b[B] = b;
+        }
+        __syncthreads();
+    }
+}
+
+/* joint outputs of the step's epilogue (csrc/arc_phys.c:739-744) */
+template <class MT>
+__device__ __forceinline__ void wave_joint_outputs(WaveWorld &W, const MT &M, const WaveK &T, int lane)
+{
+    if (lane < T.nj) {
+        int p = T.jp[lane], c = T.jc[lane];
+        Body C = W.b[c];
+        if (p >= 0) {
+            Body P = W.b[p];
+            W.angle[lane] = joint_angle<true>(&P, C, M.joint[lane]);
+            W.rate[lane] = joint_rate<true>(&P, C, M.joint[lane]);
+        } else {
+            W.angle[lane] = joint_angle<false>(nullptr, C, M.joint[lane]);
+            W.rate[lane] = joint_rate<false>(nullptr, C, M.joint[lane]);
+        }
+        W.torque[lane] = W.tacc[lane] / (float)M.substeps;
+    }
 }
 
 template <class TT>
@@ -144,80 +254,8 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
         tr.base = d.tbase[i];
         tr.outside = d.outside[i];
 
-        for (int sub = 0; sub < M.substeps; ++sub) {
-            if (lane < T.nb) {
-                Body b = W.b[lane];
-                W.p0[lane] = integrate<KGEN>(b, M.body[lane], M.g, M.h);
-                W.b[lane] = b;
-                W.ccount[lane] = 0;
-            }
-            __syncthreads();
-            for (int L = 0; L < T.nlev; ++L) {
-                int k = T.lev_first[L] + lane;
-                if (k < T.lev_first[L + 1]) {
-                    int j = T.lev_joint[k];
-                    if (T.jp[j] >= 0)
-                        wave_joint_pos<KGEN>(W, M, T, j);
-                    else
-                        wave_joint_pos<-1>(W, M, T, j);
-                }
-                __syncthreads();
-            }
-            if (T.serial_collide) {
-                /* the contact-record cap could bind: keep the global sphere order */
-                if (lane == 0) {
-                    int total = 0;
-                    for (int g = 0; g < T.ncb; ++g)
-                        wave_collide_group(W, M, T, g, tr, total);
-                }
-            } else if (lane < T.ncb) {
-                int total = 0; /* the cap cannot bind: groups are independent */
-                wave_collide_group(W, M, T, lane, tr, total);
-            }
-            __syncthreads();
-            if (lane < T.nb) {
-                Body b = W.b[lane];
-                velocities(b, W.p0[lane], M.inv_h, M.two_h);
-                W.b[lane] = b;
-            }
-            __syncthreads();
-            for (int L = 0; L < T.nlev; ++L) {
-                int k = T.lev_first[L] + lane;
-                if (k < T.lev_first[L + 1]) {
-                    int j = T.lev_joint[k];
-                    if (T.jp[j] >= 0)
-                        wave_joint_vel<KGEN>(W, M, T, j);
-                    else
-                        wave_joint_vel<-1>(W, M, T, j);
-                }
-                __syncthreads();
-            }
-            if (lane < T.ncb) {
-                const int B = T.cb_body[lane];
-                Body b = W.b[B];
-                int n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
-                for (int k = 0; k < n; ++k) {
-                    const float *c = W.cl[T.cb_slot[lane] + k];
-                    contact_velocity<KGEN>(b, M.body[B], v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]));
-                }
-                W.b[B] = b;
-            }
-            __syncthreads();
-        }
-        /* epilogue: outputs, time, stability */
-        if (lane < T.nj) {
-            int p = T.jp[lane], c = T.jc[lane];
-            Body C = W.b[c];
-            if (p >= 0) {
-                Body P = W.b[p];
-                W.angle[lane] = joint_angle<true>(&P, C, M.joint[lane]);
-                W.rate[lane] = joint_rate<true>(&P, C, M.joint[lane]);
-            } else {
-                W.angle[lane] = joint_angle<false>(nullptr, C, M.joint[lane]);
-                W.rate[lane] = joint_rate<false>(nullptr, C, M.joint[lane]);
-            }
-            W.torque[lane] = W.tacc[lane] / (float)M.substeps;
-        }
+        wave_substeps(W, M, T, GridCells{ M.terr, tr }, lane);
+        wave_joint_outputs(W, M, T, lane);
         bool bad = lane < T.nb && body_unstable(W.b[lane]);
         bool any_bad = __any(bad);
         __syncthreads();
@@ -249,10 +287,20 @@ __global__ __launch_bounds__(64) void k_step_wave(DevWorlds d, const Model *__re
             /* the last substep's contacts, for the host snapshot: copied from
              * LDS into the one-world-per-lane form's list layout */
             const int B = T.cb_body[lane], ci = cbody_index(B);
-            const int n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
-            for (int k = 0; k < n; ++k)
-                for (int f = 0; f < 6; ++f)
-                    d.cbuf[((size_t)(M.cslot[ci] + k) * 6 + f) * S + i] = W.cl[T.cb_slot[lane] + k][f];
+            int n = 0;
+            if (T.serial_collide) {
+                for (int k = 0; k < W.ntotal && n < M.ccap[ci]; ++k)
+                    if (W.cbody[k] == B) {
+                        for (int f = 0; f < 6; ++f)
+                            d.cbuf[((size_t)(M.cslot[ci] + n) * 6 + f) * S + i] = W.cl[k][f];
+                        ++n;
+                    }
+            } else {
+                n = W.ccount[B] < T.cb_cap[lane] ? W.ccount[B] : T.cb_cap[lane];
+                for (int k = 0; k < n; ++k)
+                    for (int f = 0; f < 6; ++f)
+                        d.cbuf[((size_t)(M.cslot[ci] + k) * 6 + f) * S + i] = W.cl[T.cb_slot[lane] + k][f];
+            }
             d.ncontact[(size_t)ci * S + i] = (uint8_t)n;
         }
         for (int off = 32; off > 0; off >>= 1)
@@ -280,6 +328,105 @@ hipError_t launch_step_wave(const DevWorlds &d, const Model *m_dev, const WaveK 
         hipLaunchKernelGGL(k_step_wave<double>, dim3(grid_blocks), dim3(64), 0, s, d, m_dev, wk_dev, targets_d, d.n);
     else
         hipLaunchKernelGGL(k_step_wave<float>, dim3(grid_blocks), dim3(64), 0, s, d, m_dev, wk_dev, targets_f, d.n);
+    return hipGetLastError();
+}
+
+/* columns of a terrain stored as binary32 heights, nx * ny row-major */
+struct FloatCells {
+    const TerrainK &t;
+    const float *h;
+    float outside;
+    __device__ float operator()(int ix, int iy) const
+    {
+        if (ix < 0 || iy < 0 || ix >= t.nx || iy >= t.ny)
+            return outside;
+        return h[ix + iy * t.nx];
+    }
+};
+
+/* `steps` steps of one hand-built world: the reference's arc_world_step
+ * (csrc/arc_phys.c:707-747) on the world's own joint targets, unclamped, with
+ * no skip of an unstable world; the state stays in LDS between steps */
+__global__ __launch_bounds__(64) void k_world_step(WorldState *__restrict__ st, const WorldModel *__restrict__ m,
+                                                   const WaveK *__restrict__ wk, const float *__restrict__ heights,
+                                                   int steps)
+{
+    __shared__ WaveWorld W;
+    const WorldModel &M = *m;
+    const WaveK &T = *wk;
+    const int lane = threadIdx.x;
+    if (lane < T.nb) {
+        Body &b = W.b[lane];
+        const Body &s = st->b[lane];
+        b.x = s.x;
+        b.q = s.q;
+        b.v = s.v;
+        b.w = s.w;
+    }
+    if (lane < T.nj)
+        W.target[lane] = st->target[lane];
+    if (lane < T.ns) {
+        W.touch[lane] = st->touch[lane];
+        W.anchored[lane] = st->anchored[lane];
+        W.anc[lane] = st->anc[lane];
+        W.ancn[lane] = st->ancn[lane];
+    }
+    float time = st->time;
+    int unstable = st->unstable, ov = 0;
+    const FloatCells cells{ M.terr, heights, M.outside };
+    for (int k = 0; k < steps; ++k) {
+        /* prologue: the caller may have changed orientations since the last step */
+        if (lane < T.nb) {
+            Body t = W.b[lane];
+            bsync<KGEN>(t, M.body[lane]);
+            W.b[lane] = t;
+        }
+        if (lane < T.ns)
+            W.touch[lane] = 0;
+        if (lane < T.nj)
+            W.tacc[lane] = 0.0f;
+        __syncthreads();
+        wave_substeps(W, M, T, cells, lane);
+        wave_joint_outputs(W, M, T, lane);
+        time = time + M.dt;
+        if (__any(lane < T.nb && body_unstable(W.b[lane])))
+            unstable = 1;
+        if (__any(lane < T.ncb && W.ccount[T.cb_body[lane]] > T.cb_cap[lane]))
+            ov = 1;
+        __syncthreads();
+    }
+    if (lane < T.nb) {
+        Body &s = st->b[lane];
+        const Body &b = W.b[lane];
+        s.x = b.x;
+        s.q = b.q;
+        s.v = b.v;
+        s.w = b.w;
+    }
+    if (lane < T.nj) {
+        st->angle[lane] = W.angle[lane];
+        st->rate[lane] = W.rate[lane];
+        st->torque[lane] = W.torque[lane];
+    }
+    if (lane < T.ns) {
+        st->touch[lane] = W.touch[lane];
+        st->anchored[lane] = W.anchored[lane];
+        st->anc[lane] = W.anc[lane];
+        st->ancn[lane] = W.ancn[lane];
+    }
+    if (lane == 0) {
+        st->time = time;
+        st->unstable = unstable;
+        st->overflow |= ov;
+    }
+}
+
+hipError_t launch_world_step(WorldState *st, const WorldModel *m, const WaveK *wk, const float *heights, int steps,
+                             hipStream_t s)
+{
+    if (steps <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_world_step, dim3(1), dim3(64), 0, s, st, m, wk, heights, steps);
     return hipGetLastError();
 }
 

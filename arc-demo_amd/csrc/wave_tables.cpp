@@ -56,6 +56,7 @@ bool build_wave_tables(const HostWorld &w, float cell, WaveK &T)
         for (int k = s; k < w.ns; ++k)
             if (w.s[k].body == B) {
                 T.cb_sphere[sp++] = k;
+                T.s_group[k] = g;
                 cap += window_cells(w.s[k].rad, cell);
             }
         worst += cap;
@@ -65,8 +66,12 @@ bool build_wave_tables(const HostWorld &w, float cell, WaveK &T)
         slot += cap;
     }
     T.cb_first[T.ncb] = sp;
+    /* where every window's worst case fits the cap, each group gets its own
+     * slot range (at most MAXC in all); otherwise the records share one list
+     * of MAXC, in the world's sphere order */
     T.serial_collide = worst > MAXC;
-    return slot <= WAVE_MAX_SLOTS;
+    static_assert(WAVE_MAX_SLOTS >= MAXC, "the serial list holds the reference's cap");
+    return T.serial_collide || slot <= WAVE_MAX_SLOTS;
 }
 
 } /* namespace xpbd */

@@ -20,6 +20,7 @@ struct WaveK {
     int cb_first[WAVE_MAX_BODIES + 1];             /* its spheres: cb_sphere[cb_first[g] .. cb_first[g+1]) */
     int cb_sphere[WAVE_MAX_SPHERES];
     int cb_slot[WAVE_MAX_BODIES], cb_cap[WAVE_MAX_BODIES];
+    int s_group[WAVE_MAX_SPHERES];                 /* contact group of each sphere */
     int serial_collide;                            /* the 128-record cap could bind: keep sphere order */
 };
 

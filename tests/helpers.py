@@ -237,3 +237,71 @@ class HostCheck:
 
     def error(self, i):
         return self.L.hc_error(self.p, i)
+
+
+class TwinWorld:
+    """A device world (arc.native.World) and an FP32-oracle world given the
+    same calls; each read returns the device's value once it is asserted to be
+    the oracle's, bit for bit."""
+
+    def __init__(self):
+        from arc import native
+        self.g, self.o = native.World(), P.Lib("o32").world()
+
+    def __getattr__(self, name):
+        fg, fo = getattr(self.g, name), getattr(self.o, name)
+
+        def call(*args, **kw):
+            rg, ro = fg(*args, **kw), fo(*args, **kw)
+            if rg is not None or ro is not None:
+                assert_same(rg, ro, f"{name}{args}")
+            return rg
+
+        return call
+
+    @property
+    def unstable(self):
+        assert bool(self.g.unstable) == bool(self.o.unstable)
+        return self.g.unstable
+
+    @property
+    def time(self):
+        assert_same(self.g.time, self.o.time, "time")
+        return self.g.time
+
+
+class TwinSim:
+    """A device sim (arc.native.Sim, in `form`) and an FP32-oracle sim in
+    lockstep, with the oracle front end's method signatures; observe()
+    asserts every output equal to the oracle's."""
+
+    def __init__(self, n, grid=64, cell=0.4, form=None):
+        from arc import native
+        self.g = native.Sim(n, grid=grid, cell=cell)
+        if form:
+            self.g.set_form(form)
+        self.o = P.Lib("o32").sim(n, grid=grid, cell=cell)
+
+    def robot_info(self):
+        d, lo, hi, misc = self.o.robot_info()
+        assert np.array_equal(self.g.default_angles, d) and np.array_equal(self.g.lo, lo)
+        assert np.array_equal(self.g.hi, hi) and self.g.stand_height == misc[0]
+        return d, lo, hi, misc
+
+    def reset(self, ids, seeds, difficulty):
+        ids = np.asarray(ids, np.int32)
+        self.g.reset(ids, np.asarray(seeds, np.uint32), np.asarray(difficulty, np.float64))
+        self.o.reset(ids, seeds, difficulty)
+
+    def step(self, targets, threads: int = 1):
+        self.g.step(np.ascontiguousarray(targets, np.float64))
+        self.o.step(targets, threads)
+
+    def observe(self):
+        root, joints, contacts, flags = (np.array(a) for a in self.g.observe())
+        rot = np.array(self.g.rot)
+        ref = self.o.observe()
+        for name, a, b in zip(("root", "rot", "joints", "contacts", "flags"), (root, rot, joints, contacts, flags),
+                              ref):
+            assert_same(a, b, f"observe {name}")
+        return root, rot, joints, contacts, flags

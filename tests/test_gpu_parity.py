@@ -215,3 +215,26 @@ def test_other_timesteps_and_grids(native, form, dt, substeps, grid, cell):
         g.step(ts[k])
         o.step(ts[k])
         assert_same(_states(g, n), _states(o, n), f"{form}, dt {dt}, {substeps} substeps, step {k + 1}")
+
+
+@pytest.mark.parametrize("form", ["lane", "wave"])
+def test_fine_cells(native, form):
+    """Cells of 10 cm: the robot's sphere windows add up to more than the
+    128-record contact cap, so records follow the world's sphere order under
+    one running total (per-body lists in the lane form, a single list in the
+    plain form); the group forms, which need every window within the cap,
+    refuse such a robot."""
+    n = 48
+    diff = P.difficulties(3, n)
+    g = native.Sim(n, grid=64, cell=0.1)
+    g.set_form(form)
+    o = P.Lib("o32").sim(n, grid=64, cell=0.1)
+    ids = np.arange(n, dtype=np.int32)
+    g.reset(ids, ids.astype(np.uint32), diff)
+    o.reset(ids, ids.astype(np.uint32), diff)
+    for k, t in enumerate(action_stream(n, 5, seed=6)):
+        g.step(t)
+        o.step(t)
+        assert_same(_states(g, n), _states(o, n), f"{form}, cell 0.1, step {k + 1}")
+    with pytest.raises((native.SimError, ValueError)):
+        g.set_form("leg")

@@ -1,13 +1,18 @@
-"""The reference's own known-answer tests and invariants, restated against
-the oracle: csrc/tests/test_phys.c (analytic physics cases) and
-csrc/tests/test_robot.c (standing, random actions, determinism, void).
+"""The reference's own known-answer tests and invariants: csrc/tests/test_phys.c
+(analytic physics cases) and csrc/tests/test_robot.c (standing, random
+actions, determinism, void), each run on
 
-In FP64 every check uses the reference's tolerance.  The FP32 restatement
-(the precision the HIP engine computes in) passes the same checks except where
-the reference's tolerance is below binary32 resolution (1e-9 / 1e-12 on
-quantities of order 1, and vz after 600 substeps of position-derived
-velocity); those use the tolerance stated next to them.  The reference's own
-FP32 build shows the same six failures (oracle/Makefile `make ref` builds it).
+- "o64": the FP64 oracle, with the reference's tolerances;
+- "o32": the FP32 oracle (the precision the HIP engine computes in), which
+  passes the same checks except where the reference's tolerance is below
+  binary32 resolution (1e-9 / 1e-12 on quantities of order 1, and vz after
+  600 substeps of position-derived velocity); those use the tolerance stated
+  next to them.  The reference's own FP32 build shows the same six failures
+  (oracle/Makefile `make ref` builds it);
+- "gpu" (marked gpu): the HIP engine -- the single-world entry points of
+  arc_gpu_world.h for the physics cases, the arc_sim ABI for the robot -- with
+  the FP32 tolerances, driven in lockstep with the FP32 oracle: every value a
+  check reads is also asserted equal, bit for bit, to the oracle's.
 """
 
 import math
@@ -16,13 +21,32 @@ import numpy as np
 import pytest
 
 import pyoracle as P
+from helpers import TwinSim, TwinWorld
 
 DT, SUB = 1.0 / 30.0, 20
-PREC = ["o64", "o32"]
+PREC = ["o64", "o32", pytest.param("gpu", marks=pytest.mark.gpu)]
 
 
 def tol(kind, t64, t32):
     return t64 if kind == "o64" else t32
+
+
+class GpuLib:
+    """The "gpu" kind: worlds and sims twinned with the FP32 oracle; the
+    inertia helpers are the FP32 oracle's (host-side arithmetic)."""
+
+    def world(self):
+        return TwinWorld()
+
+    def sim(self, n, grid=64, cell=0.4):
+        return TwinSim(n, grid=grid, cell=cell)
+
+    def inertia(self, *args):
+        return P.Lib("o32").inertia(*args)
+
+
+def lib(kind):
+    return GpuLib() if kind == "gpu" else P.Lib(kind)
 
 
 def empty_world(L):
@@ -38,7 +62,7 @@ def run(w, seconds):
 @pytest.mark.parametrize("kind", PREC)
 def test_free_fall(kind):
     """test_phys.c:23-35: z(1 s) = -g/2 within the symplectic-Euler error."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     w.add_body(2.0, L.inertia(0, 2.0, 0.1), (0, 0, 0))
     st = w.body(0)
@@ -54,7 +78,7 @@ def test_free_fall(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_free_rotation(kind):
     """test_phys.c:37-59: angular momentum drift <= 2 %, no energy gain."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     w.gravity(0, 0, 0)
     w.add_body(1.0, (1.0, 2.0, 3.0), (0, 0, 0))
@@ -86,7 +110,7 @@ def _rot(q):
 @pytest.mark.parametrize("kind", PREC)
 def test_pendulum(kind):
     """test_phys.c:61-101: small-angle period 2 pi sqrt(I / (m g l)) within 0.2 %."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     ln, m, r, th0 = 1.0, 1.0, 0.05, 0.1
     w.add_body(m, L.inertia(0, m, r), (0, 0, -ln))
@@ -118,7 +142,7 @@ def test_pendulum(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_chain(kind):
     """test_phys.c:103-131: 6-link chain, 10:1 masses, joints stay within 2 mm."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     n, seg = 6, 0.3
     half = math.sqrt(0.5)
@@ -155,7 +179,7 @@ def _chain_sep(bodies, seg):
 @pytest.mark.parametrize("kind", PREC)
 def test_limits(kind):
     """test_phys.c:133-154: overshoot <= 5e-3 rad, rests on the upper limit."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     half = math.sqrt(0.5)
     w.add_body(1.0, L.inertia(1, 1.0, 0.03, 0.25), (0.25, 0, 0), (half, 0, half, 0))
@@ -175,7 +199,7 @@ def test_limits(kind):
 def test_drive(kind):
     """test_phys.c:156-186: PD drive reaches its target, static sag
     m g (L/2) / kp, holding torque, saturation of a weak motor."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     w.gravity(0, 0, 0)
     m, ln, target = 1.0, 0.5, 0.5
@@ -202,7 +226,7 @@ def test_drive(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_resting_sphere(kind):
     """test_phys.c:188-206: rests at height r, still, touching every substep."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = L.world()
     w.terrain_flat(8, 8, 0.5, 0.0)
     r = 0.1
@@ -222,7 +246,7 @@ def test_resting_sphere(kind):
 def test_sliding(kind):
     """test_phys.c:217-249: stopping distance v^2 / (2 mu g); static hold on
     a 20 degree tilt; slides once mu < tan(20 deg)."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = L.world()
     v0, mu = 3.0, 0.5
     w.mu(mu, mu)
@@ -252,7 +276,7 @@ def test_sliding(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_blocks(kind):
     """test_phys.c:251-283: lookups, seam, step edge and edge-resting spheres."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = L.world()
     w.terrain_flat(8, 8, 0.5, 0.0)
     for i in range(64):
@@ -275,7 +299,7 @@ def test_blocks(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_set_pose(kind):
     """test_phys.c:285-302: forward kinematics reads back its angles."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = empty_world(L)
     w.add_body(5.0, (1, 1, 1), (0, 0, 0))
     w.add_body(1.0, (.1, .1, .1), (0.2, 0, -0.2))
@@ -294,7 +318,7 @@ def test_set_pose(kind):
 @pytest.mark.parametrize("bad", [1e12, -1e300, float("nan")])
 def test_blow_up(kind, bad):
     """test_phys.c:304-324: far or NaN states are flagged, lookups stay safe."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = L.world()
     w.terrain_flat(8, 8, 0.5, 0.0)
     w.add_body(1.0, L.inertia(0, 1.0, 0.1), (bad, 0.0, 0.05))
@@ -307,7 +331,7 @@ def test_blow_up(kind, bad):
 @pytest.mark.parametrize("kind", PREC)
 def test_void(kind):
     """test_robot.c:168-188: a ball rolled off the generated terrain falls."""
-    L = P.Lib(kind)
+    L = lib(kind)
     w = L.world()
     w.terrain_generate(32, 0.4, 0.5, 9)
     info = w.terrain_info()
@@ -327,7 +351,7 @@ def test_void(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_stand(kind):
     """test_robot.c:70-99: the quadruped settles on flat ground, all feet down."""
-    L = P.Lib(kind)
+    L = lib(kind)
     s = L.sim(1, grid=32)
     d, lo, hi, misc = s.robot_info()
     s.reset([0], [1], [0.0])
@@ -348,17 +372,17 @@ def test_stand(kind):
 @pytest.mark.parametrize("kind", PREC)
 def test_random_actions_stable(kind):
     """test_robot.c:101-146: random full-range targets every 3 steps on
-    difficulty-1 terrain: nothing blows up, torso speed < 6 m/s (shortened
-    from 16 worlds x 600 steps to 8 x 300)."""
-    L = P.Lib(kind)
-    n = 8
+    difficulty-1 terrain: nothing blows up, torso speed < 6 m/s.  The
+    reference's 16 worlds x 600 steps on the GPU; 8 x 300 on the oracles."""
+    L = lib(kind)
+    n, steps = (16, 600) if kind == "gpu" else (8, 300)
     s = L.sim(n, grid=48)
     d, lo, hi, _ = s.robot_info()
     s.reset(np.arange(n), np.arange(n) + 100, np.ones(n))
-    u = P.Lib("o64").rng_uniform(7, n * 12 * 100)
+    u = P.Lib("o64").rng_uniform(7, n * 12 * (steps // 3))
     maxspeed, unstable = 0.0, 0
     t = np.tile(d, (n, 1))
-    for k in range(300):
+    for k in range(steps):
         if k % 3 == 0:
             uu = u[(k // 3) * n * 12:(k // 3 + 1) * n * 12].reshape(n, 12)
             t = lo + (hi - lo) * uu
@@ -372,16 +396,24 @@ def test_random_actions_stable(kind):
 
 @pytest.mark.parametrize("kind", PREC)
 def test_thread_determinism(kind):
-    """test_robot.c:148-166: 1 vs 8 threads give identical states."""
-    L = P.Lib(kind)
-    a, b = L.sim(8, grid=48), L.sim(8, grid=48)
+    """test_robot.c:148-166: 1 vs 8 threads give identical states.  On the
+    GPU the counterpart of the thread count is the form of the step kernel:
+    one world per lane, per wavefront, four or two lanes per world, shared
+    roots -- all give identical states, each the FP32 oracle's."""
+    L = lib(kind)
+    if kind == "gpu":
+        sims = [TwinSim(8, grid=48, form=f) for f in ("lane", "wave", "leg", "pair", "shared")]
+    else:
+        sims = [L.sim(8, grid=48), L.sim(8, grid=48)]
     ids = np.arange(8)
-    for s in (a, b):
+    for s in sims:
         s.reset(ids, ids + 5, np.full(8, 0.7))
     u = P.Lib("o64").rng_uniform(3, 8 * 12 * 40)
-    d, lo, hi, _ = a.robot_info()
+    d, lo, hi, _ = sims[0].robot_info()
     for k in range(120):
         t = lo + (hi - lo) * u[(k // 3) * 96:(k // 3 + 1) * 96].reshape(8, 12)
-        a.step(t, 1)
-        b.step(t, 8)
-    assert np.array_equal(a.observe()[0], b.observe()[0])
+        for i, s in enumerate(sims):
+            s.step(t, 1 if i == 0 else 8)
+    roots = [s.observe()[0] for s in sims]
+    for r in roots[1:]:
+        assert np.array_equal(roots[0], r)
