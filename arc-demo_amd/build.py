@@ -33,7 +33,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
 STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
 
 HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip",
-               "noalloc.hip"]
+               "noalloc.hip", "selfcheck.hip"]
 HOST_SOURCES = ["sim.cpp", "batch.cpp", "server.cpp", "task.cpp", "model.cpp", "setup.cpp", "wave_tables.cpp",
                 "visuals.cpp", "world.cpp"]
 

@@ -49,6 +49,23 @@ XD float len2(V3 a) { return dot(a, a); }
 XD float len(V3 a) { return sqrtf(dot(a, a)); }
 XD V3 zero3() { return v3(0.0f, 0.0f, 0.0f); }
 
+/* 1 / x correctly rounded.  On the device, for 2^-126 <= |x| < 2^126, one
+ * Newton step on the hardware reciprocal: r + r (1 - x r) with fused
+ * multiply-adds equals the IEEE quotient 1.0f / x for every such float (both
+ * signs, all 2^32 bit patterns checked exhaustively on gfx950), in 3
+ * instructions instead of the 10 of the general division; every other x
+ * (zero, denormal, huge, infinite, NaN) takes the division. */
+XD float recipf(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    if (fabsf(x) >= 0x1p-126f && fabsf(x) < 0x1p126f) {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+    }
+#endif
+    return 1.0f / x;
+}
+
 /* binary64 evaluation rounded to binary32 */
 XD float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 XD float f_sin(float x) { return (float)sin((double)x); }
@@ -57,7 +74,7 @@ XD float f_cos(float x) { return (float)cos((double)x); }
 XD V3 unit(V3 a)
 {
     float l = len(a);
-    return l > 1e-15f ? scl(a, 1.0f / l) : zero3();
+    return l > 1e-15f ? scl(a, recipf(l)) : zero3();
 }
 XD V3 perp(V3 n)
 {
@@ -80,7 +97,7 @@ XD Q4 qunit(Q4 a)
     float l = sqrtf(a.w * a.w + a.x * a.x + a.y * a.y + a.z * a.z);
     if (l < 1e-15f)
         return qid();
-    l = 1.0f / l;
+    l = recipf(l);
     return q4(a.w * l, a.x * l, a.y * l, a.z * l);
 }
 XD Q4 qaxis(V3 axis, float angle)
@@ -108,7 +125,8 @@ XD Q4 qturn(Q4 q, V3 w)
 
 XD void m_from_q(M3 &r, Q4 q)
 {
-    float k = 2.0f / (q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+    /* 2 / s = 2 (1 / s) exactly: a power-of-two scale commutes with rounding */
+    float k = 2.0f * recipf(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
     float x2 = q.x * k, y2 = q.y * k, z2 = q.z * k;
     float xx = q.x * x2, yy = q.y * y2, zz = q.z * z2;
     float xy = q.x * y2, xz = q.x * z2, yz = q.y * z2;
@@ -229,7 +247,7 @@ template <int KIND> XD float pos_static(Body &b, const BodyK &k, V3 r, V3 d)
     float c = len(d);
     if (c < 1e-12f)
         return 0.0f;
-    V3 n = scl(d, 1.0f / c);
+    V3 n = scl(d, recipf(c));
     float den = wpos<KIND>(b, k, r, n);
     if (den <= 0.0f)
         return 0.0f;
@@ -248,7 +266,7 @@ XD bool sym3(const S3 &K, V3 d, V3 &x)
     float det = a * A + b * B + c * C;
     if (fabsf(det) < 0.0f)
         return false;
-    float inv = 1.0f / det;
+    float inv = recipf(det);
     float D = a * g - c * c;
     float E = b * c - a * f;
     float F = a * e - b * b;
@@ -370,7 +388,7 @@ XD void joint_positions(Body *P, const BodyK *pk, Body &C, const BodyK &ck, cons
         V3 p = zero3();
         float th = len(rot);
         if (!(th < 1e-12f)) {
-            V3 n = scl(rot, 1.0f / th);
+            V3 n = scl(rot, recipf(th));
             float den = (HP ? wrot<PKK>(*P, *pk, n) : 0.0f) + wrot<CK>(C, ck, n);
             den = den + j.soft;
             if (!(den <= 0.0f)) {
@@ -575,7 +593,7 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Ancho
     if (l < 1e-12f)
         return;
     V3 r = sub(p, b.x);
-    float wt = wpos<KIND>(b, k, r, scl(e, 1.0f / l));
+    float wt = wpos<KIND>(b, k, r, scl(e, recipf(l)));
     float lt = l / wt;
     if (lt > mus * ln) {
         frac = minf_(mud * ln / lt, 1.0f);
@@ -623,7 +641,7 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
             if (dist > 1e-9f) {
                 if (dist >= rad)
                     continue;
-                n = scl(diff, 1.0f / dist);
+                n = scl(diff, recipf(dist));
                 depth = rad - dist;
             } else {
                 n = v3(0.0f, 0.0f, 1.0f);
@@ -681,7 +699,7 @@ XD V3 gyro(const BodyK &k, V3 w0, float h)
     float det = J00 * c00 + J01 * c01 + J02 * c02;
     if (fabsf(det) < 0.0f)
         return w0;
-    float inv = 1.0f / det;
+    float inv = recipf(det);
     V3 k0 = v3(J00, J10, J20), k1 = v3(J01, J11, J21), k2 = v3(J02, J12, J22);
     V3 dw;
     dw.x = dot(f, cross(k1, k2)) * inv;

@@ -25,6 +25,12 @@ void arcx_clear_error(arc_sim *s);
 /* Number of HIP devices visible (0 when the library cannot run). */
 int arcx_device_count(void);
 
+/* Self-check of the engine's correctly rounded reciprocal on the current
+ * GPU: every binary32 bit pattern x, the step's fast 1 / x against the IEEE
+ * quotient; writes the number of differing results (0 expected).  Returns 0,
+ * or -1 on a HIP error. */
+int arcx_check_recip(unsigned long long *mismatches);
+
 /* n successive outputs of the engine's xorshift128 generator seeded `seed`
  * (the stream the reset draws from; also used to make synthetic inputs). */
 void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out);

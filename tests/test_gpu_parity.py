@@ -14,6 +14,8 @@ two, three and ten control steps; longer rollouts are compared in
 test_gpu_rollout.py.
 """
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -47,6 +49,15 @@ def test_library_loads_native(native):
     lib = native.sim_lib()
     assert lib.arcx_device_count() >= 1
     assert lib.arc_sim_num_joints() == 12 and lib.arc_sim_num_legs() == 4
+
+
+def test_fast_reciprocal_exhaustive(native):
+    """The step's 1 / x (one Newton step on the hardware reciprocal where that
+    is exact, the IEEE division elsewhere) equals the IEEE quotient for every
+    one of the 2^32 binary32 bit patterns."""
+    bad = ctypes.c_ulonglong(12345)
+    assert native.sim_lib().arcx_check_recip(ctypes.addressof(bad)) == 0
+    assert bad.value == 0
 
 
 def test_template_matches_oracle(native):
