@@ -10,9 +10,10 @@
  *     root 0, root 1, root 2, root 3, then every leg's joint 1 and joint 2
  * applies the same sequence of updates to every body (operations on disjoint
  * bodies commute exactly) and gives bit-identical results.  The four roots
- * run one after another, each in the lane holding its leg, and the torso is
- * broadcast to the group (DPP) when the next root belongs to another lane;
- * the mid and distal joints of the group's legs run at once.  Spheres,
+ * run one after another (with P = 4 each by two lanes of the quad, the coxa's
+ * and a partner holding the torso, RootsPair; with P = 2 in the lane holding
+ * its leg) and the torso is broadcast to the group (DPP) after each; the mid
+ * and distal joints of the group's legs run at once.  Spheres,
  * integration and the velocity passes act on one body at a time: each lane
  * does its legs' bodies and the torso's (the same arithmetic in every lane).
  *
@@ -25,6 +26,7 @@
  */
 #pragma once
 #include "lane_engine.h"
+#include "root_split.h"
 
 namespace xpbd {
 
@@ -184,6 +186,47 @@ template <int K> struct RootsLocal {
             if constexpr (l == K - 1)
                 bcast_vel<P, owner>(W.T);
         });
+    }
+};
+
+/* The four root joints of the quad's world (one leg per lane, K = 1), each
+ * solved by two lanes (root_split.h): root t by lane t (its coxa, the child
+ * side) and lane t ^ 1 (the torso, the parent side), so half the quad works
+ * on the roots instead of a quarter.  Lane t moves its coxa into its torso
+ * registers for the solve and takes it back afterwards; every lane's torso is
+ * then replaced by the parent side's result (DPP), which also restores lane
+ * t's.  The velocity pass stays with the owner lane (RootsLocal). */
+struct RootsPair {
+    __device__ __forceinline__ static void swap_in(bool child, Body &T, const Body &C)
+    {
+        T.x = sel3(child, C.x, T.x);
+        T.q = q4(child ? C.q.w : T.q.w, child ? C.q.x : T.q.x, child ? C.q.y : T.q.y, child ? C.q.z : T.q.z);
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            T.R.e[e / 3][e % 3] = child ? C.R.e[e / 3][e % 3] : T.R.e[e / 3][e % 3];
+    }
+    __device__ __forceinline__ void positions(LegWorld<1> &W, int q, const Model &M)
+    {
+        static_for<0, LEGS>([&](auto t) {
+            constexpr int h = t ^ 1;
+            const bool child = q == t;
+            Body &C = W.b[0][0];
+            swap_in(child, W.T, C);
+            const float tg = gbcast<4, t>(W.target[0]);
+            const float d = root_positions_pair<3 * t, QuadX<t, h>>(W.T, child, tg, M.hh, M);
+            if (child) {
+                W.tacc[0] = W.tacc[0] - d;
+                C.x = W.T.x;
+                C.q = W.T.q;
+                C.R = W.T.R;
+            }
+            bcast_pose<4, h>(W.T);
+        });
+    }
+    __device__ __forceinline__ void velocities(LegWorld<1> &W, int q, const Model &M)
+    {
+        RootsLocal<1> local;
+        local.velocities(W, q, M);
     }
 };
 

@@ -123,6 +123,20 @@ template <int BLOCK, int TILE> struct LdsTileCells {
             return dequant(r, lt[(u + v * TILE) * BLOCK]);
         return terrain_cell(t, r, ix, iy);
     }
+    /* a window of at most 2 x 2 columns inside the tile: true if no column's
+     * top is above lo (the columns right of and above a one-column window are
+     * read but not used); false when the window is not of that kind */
+    __device__ __forceinline__ bool clear(int ix0, int ix1, int iy0, int iy1, float lo) const
+    {
+        const unsigned u = (unsigned)(ix0 - tx), v = (unsigned)(iy0 - ty);
+        const bool wide = ix1 != ix0, tall = iy1 != iy0;
+        if (!(u < (unsigned)(TILE - 1) && v < (unsigned)(TILE - 1) && ix1 - ix0 <= 1 && iy1 - iy0 <= 1))
+            return false;
+        const int8_t *p = lt + (u + v * TILE) * BLOCK;
+        const float h00 = dequant(r, p[0]), h10 = dequant(r, p[BLOCK]);
+        const float h01 = dequant(r, p[TILE * BLOCK]), h11 = dequant(r, p[(TILE + 1) * BLOCK]);
+        return lo >= h00 && (!wide || lo >= h10) && (!tall || lo >= h01) && (!(wide && tall) || lo >= h11);
+    }
 };
 template <int BLOCK, int TILE> struct LdsTile {
     const int8_t *lt;
@@ -325,6 +339,10 @@ __device__ __forceinline__ void store_leg(const DevWorlds &d, size_t i, int q, c
     }
 }
 
+/* the root joints of the group forms: a lane pair per root with one leg per
+ * lane, the owner lane with two */
+template <int K> using LegRoots = std::conditional_t<K == 1, RootsPair, RootsLocal<K>>;
+
 /* one control step of world i by lane q of its group */
 template <int BLOCK, int TILE, int K>
 __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ row,
@@ -359,11 +377,11 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
         const LdsTile<WPB, TILE> tile = fill_tile<WPB, TILE>(s_tile + threadIdx.x / P, d, M.terr, (size_t)i,
                                                              d.x[0 * S + i], d.x[1 * S + i]);
         load_leg<K>(d, i, q, W);
-        RootsLocal<K> roots;
+        LegRoots<K> roots;
         ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile, roots);
     } else {
         load_leg<K>(d, i, q, W);
-        RootsLocal<K> roots;
+        LegRoots<K> roots;
         ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile(), roots);
     }
     store_leg<K>(d, opaque_index((size_t)i), q, W, ov);

@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <type_traits>
+#include <utility>
 
 #define XD __host__ __device__ __forceinline__
 
@@ -566,6 +568,13 @@ struct GridCells {
     XD float operator()(int ix, int iy) const { return terrain_cell(t, r, ix, iy); }
 };
 
+/* Column sources that can tell at once that no column of a sphere's window
+ * reaches a height: clear(ix0, ix1, iy0, iy1, lo) is true only if no column
+ * of the window has its top above lo (false when it cannot tell). */
+template <class C, class = void> struct has_window_test : std::false_type {};
+template <class C>
+struct has_window_test<C, std::void_t<decltype(std::declval<const C &>().clear(0, 0, 0, 0, 0.0f))>> : std::true_type {};
+
 /* anchored static / kinetic friction, arc_phys.c:512-539.  The anchor is
  * read and written through `an` (only when the sphere touches), the
  * substep-start pose fetched through pose0() only at a touch-down. */
@@ -619,6 +628,15 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
     }
     int ix0, ix1, iy0, iy1;
     sphere_window(t, c, rad, ix0, ix1, iy0, iy1);
+    if constexpr (has_window_test<Cells>::value) {
+        /* the loop below resolves a contact only with a column whose top is
+         * above the sphere's lowest point (or when that point is NaN); with
+         * none in the window it changes nothing but the anchor */
+        if (cell.clear(ix0, ix1, iy0, iy1, c.z - rad)) {
+            anchored = 0;
+            return;
+        }
+    }
     for (int iy = iy0; iy <= iy1; ++iy) {
         for (int ix = ix0; ix <= ix1; ++ix) {
             float top = cell(ix, iy);
