@@ -12,6 +12,13 @@
  * waiting for it; a completer thread waits for each batch's event in launch
  * order and stamps its requests done.  Two batches may be in flight, each
  * with its own targets buffer, so uploads overlap the running step.
+ *
+ * When a batch starts: at once when every open client has a request pending;
+ * while a batch runs, shortly before it is expected to end (its expected
+ * length is the running mean of the recent batches' device time), so that
+ * the requests arriving meanwhile are stepped together next and the device
+ * does not idle between batches; with none running, when requests stop
+ * arriving for a short gap or at the latest a window after the oldest.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -57,6 +64,7 @@ struct Flight {
     int worlds = 0;
     int buf = 0;
     bool ok = true;
+    clock_type::time_point launched;
 };
 
 } /* namespace */
@@ -86,9 +94,18 @@ struct arc_server {
     std::mutex sim_mu;             /* the sim and its stream */
     std::thread dispatcher;
     bool stop = false;
-    /* a batch waits at most this long after its first request for the other
-     * clients' requests; it starts at once when every open client has one */
+    /* with no batch in flight a batch waits at most this long after its first
+     * request for the other clients' requests, and no longer than `gap` after
+     * the last request that arrived; it starts at once when every open client
+     * has one */
     std::chrono::microseconds window{ 500 };
+    std::chrono::microseconds gap{ 100 };
+    /* while a batch runs, the next is launched this long before the running
+     * one is expected to end (the time to form and launch a batch) */
+    std::chrono::microseconds lead{ 60 };
+    clock_type::time_point last_arrival = clock_type::time_point::min();
+    clock_type::time_point last_done = clock_type::time_point::min();
+    double batch_us = 0.0; /* running mean of the batches' device time */
     long long next_ticket = 1;
     std::map<long long, double> finished; /* ticket -> latency (ms), until waited */
     std::map<long long, int> inflight;    /* ticket -> client, queued or running */
@@ -134,13 +151,29 @@ static void dispatch_loop(arc_server *sv)
                 return all ? ready > 0 && ready == open : ready > 0;
             };
             sv->work.wait(lk, [&] { return sv->stop || pending(false); });
-            if (!sv->stop && !pending(true)) {
-                /* gather more requests for up to the window after the oldest */
-                clock_type::time_point first = clock_type::time_point::max();
-                for (auto &kv : sv->clients)
-                    if (!kv.second.queue.empty())
-                        first = std::min(first, kv.second.queue.front().submitted);
-                sv->work.wait_until(lk, first + sv->window, [&] { return sv->stop || pending(true); });
+            /* gather requests until the batch's start (re-evaluated whenever a
+             * request arrives or a batch ends) */
+            while (!sv->stop && !pending(true)) {
+                clock_type::time_point when;
+                if (sv->flights.empty()) {
+                    clock_type::time_point first = clock_type::time_point::max();
+                    for (auto &kv : sv->clients)
+                        if (!kv.second.queue.empty())
+                            first = std::min(first, kv.second.queue.front().submitted);
+                    when = std::min(first + sv->window, sv->last_arrival + sv->gap);
+                } else {
+                    /* expected end of the last flight: each starts when it was
+                     * launched or when the one before it ends */
+                    const auto len = std::chrono::microseconds((long long)sv->batch_us);
+                    clock_type::time_point end = sv->last_done;
+                    for (const Flight &fl : sv->flights)
+                        end = std::max(fl.launched, end) + len;
+                    when = end - sv->lead;
+                }
+                if (clock_type::now() >= when)
+                    break;
+                const size_t nf = sv->flights.size();
+                sv->work.wait_until(lk, when, [&] { return sv->stop || pending(true) || sv->flights.size() != nf; });
             }
             if (sv->stop && !pending(false))
                 break;
@@ -195,6 +228,7 @@ static void dispatch_loop(arc_server *sv)
             for (auto &r : f.reqs)
                 if (!r.dev)
                     sv->pool.push_back(std::move(r.targets)); /* copied into the staging */
+            f.launched = clock_type::now();
             sv->flights.push_back(std::move(f));
             sv->launched++;
         }
@@ -235,12 +269,18 @@ static void complete_loop(arc_server *sv)
             sv->batches++;
             sv->requests += (long long)f.reqs.size();
             sv->world_steps += f.worlds;
+            /* the batch's device time: from its launch, or from the end of the
+             * batch before it when it was queued behind that one */
+            const double us = std::chrono::duration<double, std::micro>(t - std::max(f.launched, sv->last_done)).count();
+            sv->batch_us = sv->batch_us > 0.0 ? 0.8 * sv->batch_us + 0.2 * us : us;
+            sv->last_done = t;
             sv->flights.pop_front();
         }
         if (!ok)
             fprintf(stderr, "libarcsim: step server: batch failed (%s)\n", arcx_error_message(sv->sim));
         sv->done.notify_all();
         sv->landed.notify_all();
+        sv->work.notify_all();
     }
 }
 
@@ -449,6 +489,7 @@ long long arcs_submit(arc_server *sv, int client, const float *targets)
         r.ticket = ticket = sv->next_ticket++;
         r.client = client;
         r.submitted = t;
+        sv->last_arrival = std::max(sv->last_arrival, t);
         sv->inflight[ticket] = client;
         c.queue.push_back(std::move(r));
     }
@@ -475,6 +516,7 @@ long long arcs_submit_device(arc_server *sv, int client, const float *targets_de
         r.ticket = ticket = sv->next_ticket++;
         r.client = client;
         r.submitted = t;
+        sv->last_arrival = std::max(sv->last_arrival, t);
         sv->inflight[ticket] = client;
         it->second.queue.push_back(std::move(r));
     }

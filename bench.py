@@ -374,8 +374,9 @@ def served_pass(steps: int, warm: int) -> dict:
 
 
 def serving_pass(seconds: float) -> dict:
-    """Clients log-uniform in [1, 8192] worlds, Poisson arrivals at half the
-    saturation rate measured first in closed loop."""
+    """Clients log-uniform in [1, 8192] worlds: closed loop with one request in
+    flight per client, saturation with two in flight per client, then Poisson
+    arrivals at half the saturation rate."""
     from arc import native, server
     u = rng(4, CLIENTS).astype(np.float64) * (1.0 / 4294967296.0)
     sizes = [max(1, int(math.floor(math.exp(x * math.log(8192.0))))) for x in u]
@@ -395,22 +396,31 @@ def serving_pass(seconds: float) -> dict:
         for x in [sv.submit(c, acts(c)) for c in cids]:
             sv.wait(x)
     launch, _ = sv.autotune(TUNE_STEPS)
-    # saturation: every client keeps one request in flight
-    stop = time.perf_counter() + min(1.0, seconds / 3)
-    for c in cids:
-        sv.wait(sv.submit(c, acts(c)))
-    sv.reset_stats()
 
-    def closed(c):
-        while time.perf_counter() < stop:
+    def loop(depth):
+        """every client keeps `depth` requests in flight for a while"""
+        stop = time.perf_counter() + min(1.0, seconds / 3)
+        for c in cids:
             sv.wait(sv.submit(c, acts(c)))
+        sv.reset_stats()
 
-    th = [threading.Thread(target=closed, args=(c,)) for c in cids]
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    sat = sv.stats()
+        def closed(c):
+            q = [sv.submit(c, acts(c)) for _ in range(depth - 1)]
+            while time.perf_counter() < stop:
+                q.append(sv.submit(c, acts(c)))
+                sv.wait(q.pop(0))
+            for x in q:
+                sv.wait(x)
+
+        th = [threading.Thread(target=closed, args=(c,)) for c in cids]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        return sv.stats()
+
+    one = loop(1)  # closed loop, one request in flight per client
+    sat = loop(2)  # saturation: two in flight per client, a batch always queued
     rate = 0.5 * sat["requests"] / sat["seconds"]
     # open loop: merged Poisson arrivals, each to a uniformly chosen client
     g = np.random.default_rng(5)
@@ -432,7 +442,12 @@ def serving_pass(seconds: float) -> dict:
     sv.close_server()
     return {"clients": CLIENTS, "worlds": int(sum(sizes)), "sizes": sizes, "launch": launch,
             "saturation_requests_per_s": sat["requests"] / sat["seconds"],
-            "saturation_world_steps_per_s": sat["world_steps_per_s"], "offered_requests_per_s": rate,
+            "saturation_world_steps_per_s": sat["world_steps_per_s"],
+            "saturation_requests_per_batch": sat["requests_per_batch"],
+            "closed_loop_one_in_flight": {"requests_per_s": one["requests"] / one["seconds"],
+                                          "world_steps_per_s": one["world_steps_per_s"], "p50_ms": one["p50_ms"],
+                                          "p99_ms": one["p99_ms"], "requests_per_batch": one["requests_per_batch"]},
+            "offered_requests_per_s": rate,
             "requests": int(st["requests"]), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
             "world_steps_per_s": st["world_steps_per_s"], "requests_per_batch": st["requests_per_batch"]}
 

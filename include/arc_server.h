@@ -8,12 +8,15 @@
  *     when the worlds already held plus n fit the budget (bytes per world as
  *     arcx_bytes_per_world reports) and a contiguous free range exists;
  *   - batching: step requests are queued as they arrive; a dispatcher thread
- *     takes the oldest pending request of every client (waiting up to a
- *     batching window for the clients that have none yet), uploads their
- *     host targets (device targets are read where the client holds them)
- *     and steps all their worlds in one batched launch
- *     (arcx_step_batch with the fall rules and auto-reset, replayed from a
- *     hipGraph);
+ *     takes the oldest pending request of every client, uploads their host
+ *     targets (device targets are read where the client holds them) and
+ *     steps all their worlds in one batched launch (arcx_step_batch with the
+ *     fall rules and auto-reset, replayed from a hipGraph).  A batch starts
+ *     at once when every open client has a request pending; while another
+ *     batch runs, shortly before that one is expected to end (so requests
+ *     arriving meanwhile go together into the next); with none running, when
+ *     requests stop arriving for 100 us, at the latest a batching window
+ *     after the oldest;
  *   - latency: each request is timed from its submission to the end of the batch
  *     that stepped it (host steady clock); arcs_stats reports p50 / p99.
  * A client's requests are stepped in submission order, one per batch.
@@ -69,9 +72,10 @@ double arcs_wait(arc_server *sv, long long ticket);
  * batches; out5 / ms as arcx_autotune.  Call it once the clients are open. */
 int arcs_autotune(arc_server *sv, int steps, int *out5, double *ms);
 
-/* Batching window: a batch waits up to this long after its oldest request
- * for requests of the other open clients, and starts at once when every
- * open client has one pending (default 500 us; 0 = never wait). */
+/* Batching window: with no batch running, a batch waits at most this long
+ * after its oldest request for requests of the other open clients (less when
+ * requests stop arriving for 100 us), and starts at once when every open
+ * client has one pending (default 500 us; 0 = never wait). */
 int arcs_set_window(arc_server *sv, double microseconds);
 
 /* out[0..9]: requests completed, batches launched, world-steps, mean
