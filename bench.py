@@ -182,6 +182,9 @@ def forms_pass(steps: int, warm: int) -> dict:
             s.set_form(form)
             s.set_launch(block, 0)
             s.set_layout(16)
+            # one untimed launch first: the form's first launch also loads its code
+            s.time_steps(t.ptr + 4 * per * (k0 % S), 1, per)
+            k0 += 1
             kms = 0.0
             for k in range(steps):
                 m, _ = s.time_steps(t.ptr + 4 * per * ((k0 + k) % S), 1, per)

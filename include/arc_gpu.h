@@ -168,7 +168,8 @@ int arcx_graph_captures(const arc_sim *s);
 /* Autotuner: times every (form: lane, leg, pair or shared; threads per
  * block; LDS layout; distribution) over `steps` control steps of this
  * handle's own worlds from their current state (restored afterwards), with
- * fresh uniform actions at every step, keeps the fastest (a persistent
+ * fresh uniform actions at every step, twice (all candidates in order, then
+ * in reverse order, the two times averaged), keeps the fastest (a persistent
  * distribution only when at least 10 % faster than the best grid-stride one)
  * and reports it in out5 (as arcx_get_launch) and its ms per step in ms_out. */
 int arcx_autotune(arc_sim *s, int steps, int *out5, double *ms_out);
