@@ -148,7 +148,24 @@ struct ContactStore {
         l = v3(p[0], p[stride], p[2 * stride]);
         n = v3(p[3 * stride], p[4 * stride], p[5 * stride]);
     }
+    /* optional LDS home of the first CMIRROR records of one list (the list
+     * most contacts land in): record k field f at mirror[(k * 6 + f) * mstride] */
+    float *mirror = nullptr;
+    int mstride = 0;
+    XD void mput(int k, V3 l, V3 n) const
+    {
+        float *p = mirror + k * 6 * mstride;
+        p[0] = l.x; p[mstride] = l.y; p[2 * mstride] = l.z;
+        p[3 * mstride] = n.x; p[4 * mstride] = n.y; p[5 * mstride] = n.z;
+    }
+    XD void mget(int k, V3 &l, V3 &n) const
+    {
+        const float *p = mirror + k * 6 * mstride;
+        l = v3(p[0], p[mstride], p[2 * mstride]);
+        n = v3(p[3 * mstride], p[4 * mstride], p[5 * mstride]);
+    }
 };
+constexpr int CMIRROR = 4;
 
 /* One control step (arc_world_step, csrc/arc_phys.c:707-747).
  * Returns a bit mask of bodies whose contact list overflowed its slots

@@ -400,10 +400,20 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
             int anch = (W.anchored >> k) & 1u;
             int hit = 0;
             auto rec = [&](V3 lp, V3 n) {
-                if (ncount[NC] < leg_cap(M, KIND))
-                    cs.put(leg_slot<K>(M, KIND, l) + ncount[NC], lp, n);
-                else
+                /* a lane's first tibia records live in LDS; HBM gets the others
+                 * and, in the last substep, all of them (the host snapshot's) */
+                bool hbm = true;
+                if constexpr (KIND == 2 && K == 1)
+                    if (cs.mirror && ncount[NC] < CMIRROR) {
+                        cs.mput(ncount[NC], lp, n);
+                        hbm = sub0 == M.substeps - 1;
+                    }
+                if (ncount[NC] < leg_cap(M, KIND)) {
+                    if (hbm)
+                        cs.put(leg_slot<K>(M, KIND, l) + ncount[NC], lp, n);
+                } else {
                     overflow |= 1 << (KIND == 0 ? 0 : KIND == 1 ? 1 + q0 * K + l : 5 + q0 * K + l);
+                }
                 ncount[NC]++;
             };
             auto an = A.at(k == 0 ? 0 : 5 * (q * K + l) + 1 + m);
@@ -449,7 +459,13 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
             int n = ncount[c] < leg_cap(M, KIND) ? ncount[c] : leg_cap(M, KIND);
             for (int k = 0; k < n; ++k) {
                 V3 lp, nn;
-                cs.get(leg_slot<K>(M, KIND, l) + k, lp, nn);
+                bool lds = false;
+                if constexpr (KIND == 2 && K == 1)
+                    lds = cs.mirror && k < CMIRROR;
+                if (lds)
+                    cs.mget(k, lp, nn);
+                else
+                    cs.get(leg_slot<K>(M, KIND, l) + k, lp, nn);
                 if constexpr (KIND == 0)
                     contact_velocity<KISO>(W.T, robot::body[0], lp, nn);
                 else

@@ -180,6 +180,18 @@ __device__ __forceinline__ LdsTile<STRIDE, TILE> fill_tile(int8_t *lt, const Dev
     return T;
 }
 
+/* Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8),
+ * each XCD with its own L2.  With a grid that is a multiple of 8, workgroup b
+ * takes the slot (b mod 8) (G / 8) + b / 8 of the grid instead of b, so that
+ * each XCD steps one contiguous run of worlds and the cache lines of their
+ * structure-of-arrays state are not split between two L2s (a quarter less
+ * HBM fetched by the leg kernel at 65,536 worlds).  Any grid: a bijection. */
+__device__ __forceinline__ int xcd_slot()
+{
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    return G % 8 == 0 ? (b % 8) * (G / 8) + b / 8 : b;
+}
+
 /* An index the optimiser cannot see through.  The world's SoA addresses are
  * recomputed from it for the stores at the end of the step instead of being
  * shared with the loads at its start, which would keep ~450 64-bit addresses
@@ -257,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(DevWorlds d, const Model *__rest
             v = atomicAdd(work, 64);
         return __shfl(v, 0, 64);
     };
-    int base = work ? next(0) : (int)blockIdx.x * BLOCK + (threadIdx.x & ~63);
+    int base = work ? next(0) : xcd_slot() * BLOCK + (threadIdx.x & ~63);
     while (base < total) {
         const int t = base + lane;
         if (t < total) {
@@ -346,7 +358,7 @@ template <int K> using LegRoots = std::conditional_t<K == 1, RootsPair, RootsLoc
 /* one control step of world i by lane q of its group */
 template <int BLOCK, int TILE, int K>
 __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ row,
-                                               LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q)
+                                               LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q, float *crec)
 {
     constexpr int P = LEGS / K;
     const size_t S = d.stride;
@@ -367,6 +379,8 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
     cs.base = d.lcbuf;
     cs.stride = P * S;
     cs.world = (size_t)i * P + q;
+    cs.mirror = crec;
+    cs.mstride = BLOCK;
     cs.counts = d.ncontact;
     cs.cstride = S;
     cs.cworld = (size_t)i;
@@ -395,6 +409,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
     constexpr int P = LEGS / K;
     __shared__ float s_pose[(1 + 3 * K) * 7 * BLOCK];
     __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * (BLOCK / P) : 1];
+    __shared__ float s_crec[K == 1 ? CMIRROR * 6 * BLOCK : 1]; /* each lane's first tibia contact records */
     const Model &M = *m;
     LdsPoses<BLOCK> P0;
     P0.p = s_pose;
@@ -412,7 +427,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
             v = atomicAdd(work, 64);
         return __shfl(v, 0, 64);
     };
-    int base = work ? next(0) : (int)blockIdx.x * BLOCK + (threadIdx.x & ~63);
+    int base = work ? next(0) : xcd_slot() * BLOCK + (threadIdx.x & ~63);
     while (base < items) {
         const int t = base + lane;
         if (t < items) {
@@ -421,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
             const int i = batch ? batch_world(batch, n_ranges, w, row) : w;
             if (!batch)
                 row = targets + (size_t)w * NJ;
-            step_world_leg<BLOCK, TILE, K>(d, M, row, P0, s_tile, i, t % P);
+            step_world_leg<BLOCK, TILE, K>(d, M, row, P0, s_tile, i, t % P, K == 1 ? s_crec + threadIdx.x : nullptr);
         }
         base = next(base);
     }
@@ -497,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
         return;
     const int items = 4 * total;
     /* whole blocks move together: every lane makes the same exchanges */
-    for (int base = (int)blockIdx.x * BLOCK; base < items; base += (int)gridDim.x * BLOCK) {
+    for (int base = xcd_slot() * BLOCK; base < items; base += (int)gridDim.x * BLOCK) {
         const int t = base + (int)threadIdx.x;
         const bool valid = t < items;
         const int w = (valid ? t : items - 1) >> 2;
