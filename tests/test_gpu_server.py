@@ -102,6 +102,52 @@ def test_concurrent_clients(mods):
     sv.close_server()
 
 
+def test_pipelined_clients_share_batches(mods):
+    """Clients that keep two requests in flight: the dispatcher starts a batch
+    as soon as every open client has a request, so nearly every batch holds
+    all of them (one is always queued behind the running one), and each
+    client's worlds still evolve exactly as in a sim of their own."""
+    native, server = mods
+    sizes, steps = [512, 1024, 64, 2048], 12
+    per = native.Sim(1).bytes_per_world()
+    sv = server.StepServer(4000 * (per + 96))
+    cids = [sv.open(n, seed0=100 * i, difficulty=0.5) for i, n in enumerate(sizes)]
+    acts = [action_stream(n, steps, seed=20 + i).astype(np.float32) for i, n in enumerate(sizes)]
+    sv.reset_stats()
+    errors = []
+
+    def run(i):
+        try:
+            q = [sv.submit(cids[i], acts[i][0])]
+            for k in range(1, steps):
+                q.append(sv.submit(cids[i], acts[i][k]))
+                sv.wait(q.pop(0))
+            sv.wait(q.pop(0))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(sizes))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors
+    st = sv.stats()
+    assert st["requests"] == steps * len(sizes) and st["world_steps"] == steps * sum(sizes)
+    print(f"pipelined clients: {st['batches']:.0f} batches, {st['requests_per_batch']:.2f} requests per batch, "
+          f"p50 {st['p50_ms']:.3f} ms, p99 {st['p99_ms']:.3f} ms")
+    # the first requests may start a batch of their own; afterwards every batch holds every client's next step
+    assert st["requests_per_batch"] >= 0.75 * len(sizes)
+    for i, n in enumerate(sizes):
+        r = native.Sim(n)
+        ids = np.arange(n, dtype=np.int32)
+        r.reset(ids, (ids + 100 * i).astype(np.uint32), np.full(n, 0.5))
+        for k in range(steps):
+            r.step(acts[i][k].astype(np.float64))
+        assert np.array_equal(sv.observe(cids[i]), r.observe()[0]), f"client {i}"
+    sv.close_server()
+
+
 def test_device_submission_matches_host(mods):
     """Targets handed over in device memory (a libarcsim buffer or a torch
     tensor) step the worlds exactly as the same targets handed over from host
