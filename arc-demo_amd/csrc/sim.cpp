@@ -33,8 +33,10 @@ static bool ensure_io(arc_sim *s, size_t doubles)
 {
     if (doubles <= s->d_io_doubles)
         return true;
-    if (s->d_io)
+    if (s->d_io) {
+        (void)hipStreamSynchronize(s->stream); /* a queued step may still read it */
         (void)hipFree(s->d_io);
+    }
     s->d_io = nullptr;
     s->d_io_doubles = 0;
     hipError_t e = hipMalloc((void **)&s->d_io, doubles * sizeof(double));
@@ -65,8 +67,10 @@ static bool ensure_pin(arc_sim *s, size_t bytes)
 {
     if (bytes <= s->h_pin_bytes)
         return true;
-    if (s->h_pin)
+    if (s->h_pin) {
+        (void)hipStreamSynchronize(s->stream); /* a queued upload may still read it */
         (void)hipHostFree(s->h_pin);
+    }
     s->h_pin = nullptr;
     s->h_pin_bytes = 0;
     hipError_t e = hipHostMalloc((void **)&s->h_pin, bytes, hipHostMallocDefault);
@@ -166,6 +170,7 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
     const size_t jtab_bytes = s->su.jtab.size() * sizeof(uint32_t);
     ResetK rk_dev = s->su.rk; /* the device copy points at the device tables */
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) == hipSuccess
+        && (e = hipEventCreateWithFlags(&s->up_done, hipEventDisableTiming)) == hipSuccess
         && (e = dev_alloc(s->dev, n_worlds, s->su.grid, M.cslots)) == hipSuccess
         && (e = hipMalloc((void **)&s->d_model, sizeof(Model))) == hipSuccess
         && (e = hipMalloc((void **)&s->d_reset, sizeof(ResetK))) == hipSuccess
@@ -254,6 +259,7 @@ void arc_sim_destroy(arc_sim *s)
     if (s->d_io) (void)hipFree(s->d_io);
     if (s->d_aux) (void)hipFree(s->d_aux);
     if (s->h_pin) (void)hipHostFree(s->h_pin);
+    if (s->up_done) (void)hipEventDestroy(s->up_done);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     free(s->mirror);
     delete s;
@@ -344,15 +350,24 @@ void arc_sim_step(arc_sim *s, const double *targets)
         return;
     if (!ensure_io(s, nt) || !ensure_pin(s, nt * sizeof(float)))
         return;
-    HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
+    /* the staging buffer is free again once the previous step's upload is done
+     * (that step itself may still be running) */
+    if (s->up_pending) {
+        HIPCHK(s, hipEventSynchronize(s->up_done), "arc_sim_step");
+        s->up_pending = false;
+    }
     /* the engine reads binary32 targets; rounding before the clamp to binary32
      * limits gives the clamp of the rounded value, as the FP32 engine computes */
     float *hf = (float *)s->h_pin;
     for (size_t k = 0; k < nt; ++k)
         hf[k] = (float)targets[k];
     HIPCHK(s, hipMemcpyAsync(s->d_io, s->h_pin, nt * sizeof(float), hipMemcpyHostToDevice, s->stream), "arc_sim_step upload");
+    HIPCHK(s, hipEventRecord(s->up_done, s->stream), "arc_sim_step upload");
+    s->up_pending = true;
     HIPCHK(s, launch_form(s, (const float *)s->d_io), "arc_sim_step launch");
-    HIPCHK(s, hipStreamSynchronize(s->stream), "arc_sim_step");
+    /* returns with the step queued on the sim's stream: every entry point that
+     * reads results on the host, or rewrites what the step reads, waits for
+     * that stream first, so the next call's conversion overlaps this step */
 }
 
 void arc_sim_observe(const arc_sim *cs, double *root, double *rot, double *joints, double *contacts, int *flags)

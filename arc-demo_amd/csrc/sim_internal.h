@@ -34,6 +34,8 @@ struct arc_sim {
     size_t d_aux_bytes;
     char *h_pin;           /* pinned host staging */
     size_t h_pin_bytes;
+    hipEvent_t up_done;    /* arc_sim_step's upload out of h_pin finished */
+    bool up_pending;       /* up_done recorded and not yet waited for */
     float *d_dump;
     arc_world *mirror;
     int block, grid_blocks;

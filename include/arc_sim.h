@@ -67,7 +67,9 @@ void arc_sim_robot_info(const arc_sim *s, double *default_angles, double *lo, do
 void arc_sim_reset(arc_sim *s, const int *ids, int n_ids, const unsigned int *seeds, const double *difficulty);
 
 /* replaces csrc/arc_sim.h:42 (arc_sim.c:123-138): targets N x 12 drive targets
- * (rad), clamped to the joint limits; unstable worlds are not stepped. */
+ * (rad), clamped to the joint limits; unstable worlds are not stepped.  The
+ * targets are read before the call returns; the step may still be running
+ * on the GPU, and every function that reports state waits for it. */
 void arc_sim_step(arc_sim *s, const double *targets);
 
 /* replaces csrc/arc_sim.h:46-47 (arc_sim.c:140-183).  Any output may be NULL.
