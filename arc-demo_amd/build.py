@@ -29,8 +29,11 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
 
 # the step kernels are scheduled for instruction-level parallelism within their
 # register budget (measured ~2 % faster on MI355X at 65,536 worlds; scheduling
-# reorders independent instructions only, so the results are the same bits)
-STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
+# reorders independent instructions only, so the results are the same bits), and
+# their basic blocks are placed by the ext-TSP model, which keeps the frequent
+# paths contiguous (fewer taken branches and instruction fetches per substep;
+# layout only, same instructions)
+STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp", "-mllvm", "-enable-ext-tsp-block-placement"]
 
 HIP_SOURCES = ["step_b64.hip", "step_b128.hip", "step_b256.hip", "kernels.hip", "wave_kernel.hip", "task_kernels.hip",
                "noalloc.hip", "selfcheck.hip"]

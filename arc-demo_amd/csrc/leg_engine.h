@@ -408,7 +408,7 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
                         cs.mput(ncount[NC], lp, n);
                         hbm = sub0 == M.substeps - 1;
                     }
-                if (ncount[NC] < leg_cap(M, KIND)) {
+                if (ARC_LIKELY(ncount[NC] < leg_cap(M, KIND))) {
                     if (hbm)
                         cs.put(leg_slot<K>(M, KIND, l) + ncount[NC], lp, n);
                 } else {

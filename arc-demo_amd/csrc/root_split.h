@@ -53,7 +53,7 @@ __device__ __forceinline__ void push_pos_iso(Body &B, float invm, float invI, V3
 /* rot_block<KISO, KISO>: the parent takes p, the child -p */
 __device__ __forceinline__ void rot_block_pair(Body &B, bool child, float invI, V3 rot)
 {
-    if (len2(rot) < 1e-24f)
+    if (ARC_UNLIKELY(len2(rot) < 1e-24f))
         return;
     constexpr float iP = robot::body[0].invI.x, iC = robot::body[1].invI.x;
     S3 K = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
@@ -68,7 +68,7 @@ __device__ __forceinline__ void rot_block_pair(Body &B, bool child, float invI, 
 /* point_block<KISO, KISO> with the parent's anchor r1 and the child's r2 */
 __device__ __forceinline__ void point_block_pair(Body &B, bool child, float invm, float invI, V3 r1, V3 r2, V3 d)
 {
-    if (len2(d) < 1e-24f)
+    if (ARC_UNLIKELY(len2(d) < 1e-24f))
         return;
     S3 K = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
     point_mass<KISO>(K, B, robot::body[0], r1); /* the isotropic form reads only r and the constants */
@@ -100,12 +100,12 @@ __device__ __forceinline__ float root_positions_pair(Body &B, bool child, float 
         V3 rot = scl(a1, phi - target);
         V3 p = zero3();
         float th = len(rot);
-        if (!(th < 1e-12f)) {
+        if (ARC_LIKELY(!(th < 1e-12f))) {
             V3 n = scl(rot, recipf(th));
             const float w = dot(n, scl(n, invI)); /* wrot<KISO> of the lane's body */
             float den = x.P(w) + x.C(w);
             den = den + J.soft;
-            if (!(den <= 0.0f)) {
+            if (ARC_LIKELY(!(den <= 0.0f))) {
                 float dl = th / den;
                 if (dl > J.cap_pos)
                     dl = J.cap_pos;
@@ -119,7 +119,7 @@ __device__ __forceinline__ float root_positions_pair(Body &B, bool child, float 
         const V3 u = brot(B, sel3(child, J.b2, J.a1)), v = brot(B, J.b1);
         const V3 a1 = x.P(u), e1 = x.P(v), e2 = x.C(u);
         float s = dot(cross(e1, e2), a1), c = dot(e1, e2);
-        if (J.wide || s * J.lim[0] - c * J.lim[1] < 0.0f || J.lim[3] * c - J.lim[2] * s < 0.0f) {
+        if (ARC_UNLIKELY(J.wide || s * J.lim[0] - c * J.lim[1] < 0.0f || J.lim[3] * c - J.lim[2] * s < 0.0f)) {
             float phi = f_atan2(s, c);
             if (phi < J.lo)
                 rot_block_pair(B, child, invI, scl(a1, phi - J.lo));

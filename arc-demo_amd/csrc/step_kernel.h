@@ -119,7 +119,7 @@ template <int BLOCK, int TILE> struct LdsTileCells {
     __device__ __forceinline__ float operator()(int ix, int iy) const
     {
         unsigned u = (unsigned)(ix - tx), v = (unsigned)(iy - ty);
-        if (u < (unsigned)TILE && v < (unsigned)TILE)
+        if (ARC_LIKELY(u < (unsigned)TILE && v < (unsigned)TILE))
             return dequant(r, lt[(u + v * TILE) * BLOCK]);
         return terrain_cell(t, r, ix, iy);
     }
@@ -130,7 +130,7 @@ template <int BLOCK, int TILE> struct LdsTileCells {
     {
         const unsigned u = (unsigned)(ix0 - tx), v = (unsigned)(iy0 - ty);
         const bool wide = ix1 != ix0, tall = iy1 != iy0;
-        if (!(u < (unsigned)(TILE - 1) && v < (unsigned)(TILE - 1) && ix1 - ix0 <= 1 && iy1 - iy0 <= 1))
+        if (ARC_UNLIKELY(!(u < (unsigned)(TILE - 1) && v < (unsigned)(TILE - 1) && ix1 - ix0 <= 1 && iy1 - iy0 <= 1)))
             return false;
         const int8_t *p = lt + (u + v * TILE) * BLOCK;
         const float h00 = dequant(r, p[0]), h10 = dequant(r, p[BLOCK]);

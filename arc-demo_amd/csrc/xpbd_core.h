@@ -29,6 +29,12 @@
 #include <utility>
 
 #define XD __host__ __device__ __forceinline__
+/* Branch hints for conditions that hold in a small fraction of calls (degenerate
+ * lengths, joint limits, touch-downs, lookups outside the LDS tile).  They only
+ * move the rarely run code out of the hot instruction stream, so that the common
+ * path runs without taken branches; the arithmetic is unchanged. */
+#define ARC_LIKELY(c) __builtin_expect(!!(c), 1)
+#define ARC_UNLIKELY(c) __builtin_expect(!!(c), 0)
 
 namespace xpbd {
 
@@ -60,7 +66,7 @@ XD V3 zero3() { return v3(0.0f, 0.0f, 0.0f); }
 XD float recipf(float x)
 {
 #ifdef __HIP_DEVICE_COMPILE__
-    if (fabsf(x) >= 0x1p-126f && fabsf(x) < 0x1p126f) {
+    if (ARC_LIKELY(fabsf(x) >= 0x1p-126f && fabsf(x) < 0x1p126f)) {
         const float r = __builtin_amdgcn_rcpf(x);
         return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
     }
@@ -76,7 +82,7 @@ XD float f_cos(float x) { return (float)cos((double)x); }
 XD V3 unit(V3 a)
 {
     float l = len(a);
-    return l > 1e-15f ? scl(a, recipf(l)) : zero3();
+    return ARC_LIKELY(l > 1e-15f) ? scl(a, recipf(l)) : zero3();
 }
 XD V3 perp(V3 n)
 {
@@ -97,7 +103,7 @@ XD Q4 qmul(Q4 a, Q4 b)
 XD Q4 qunit(Q4 a)
 {
     float l = sqrtf(a.w * a.w + a.x * a.x + a.y * a.y + a.z * a.z);
-    if (l < 1e-15f)
+    if (ARC_UNLIKELY(l < 1e-15f))
         return qid();
     l = recipf(l);
     return q4(a.w * l, a.x * l, a.y * l, a.z * l);
@@ -247,11 +253,11 @@ template <int KIND> XD void push_vel(Body &b, const BodyK &k, V3 r, V3 p)
 template <int KIND> XD float pos_static(Body &b, const BodyK &k, V3 r, V3 d)
 {
     float c = len(d);
-    if (c < 1e-12f)
+    if (ARC_UNLIKELY(c < 1e-12f))
         return 0.0f;
     V3 n = scl(d, recipf(c));
     float den = wpos<KIND>(b, k, r, n);
-    if (den <= 0.0f)
+    if (ARC_UNLIKELY(den <= 0.0f))
         return 0.0f;
     float dl = c / den;
     push_pos<KIND>(b, k, r, scl(n, dl));
@@ -338,7 +344,7 @@ struct JointK {
 template <int PK, int CK>
 XD void rot_block(Body *P, const BodyK *pk, Body &C, const BodyK &ck, V3 rot)
 {
-    if (len2(rot) < 1e-24f)
+    if (ARC_UNLIKELY(len2(rot) < 1e-24f))
         return;
     S3 K = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
     if (PK >= 0)
@@ -356,7 +362,7 @@ XD void rot_block(Body *P, const BodyK *pk, Body &C, const BodyK &ck, V3 rot)
 template <int PK, int CK>
 XD void point_block(Body *P, const BodyK *pk, V3 r1, Body &C, const BodyK &ck, V3 r2, V3 d)
 {
-    if (len2(d) < 1e-24f)
+    if (ARC_UNLIKELY(len2(d) < 1e-24f))
         return;
     S3 K = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
     if (PK >= 0)
@@ -389,11 +395,11 @@ XD void joint_positions(Body *P, const BodyK *pk, Body &C, const BodyK &ck, cons
         V3 rot = scl(a1, phi - target);
         V3 p = zero3();
         float th = len(rot);
-        if (!(th < 1e-12f)) {
+        if (ARC_LIKELY(!(th < 1e-12f))) {
             V3 n = scl(rot, recipf(th));
             float den = (HP ? wrot<PKK>(*P, *pk, n) : 0.0f) + wrot<CK>(C, ck, n);
             den = den + j.soft;
-            if (!(den <= 0.0f)) {
+            if (ARC_LIKELY(!(den <= 0.0f))) {
                 float dl = th / den;
                 if (dl > j.cap_pos)
                     dl = j.cap_pos;
@@ -410,7 +416,7 @@ XD void joint_positions(Body *P, const BodyK *pk, Body &C, const BodyK &ck, cons
         V3 e1 = HP ? brot(*P, j.b1) : j.b1;
         V3 e2 = brot(C, j.b2);
         float s = dot(cross(e1, e2), a1), c = dot(e1, e2);
-        if (j.wide || s * j.lim[0] - c * j.lim[1] < 0.0f || j.lim[3] * c - j.lim[2] * s < 0.0f) {
+        if (ARC_UNLIKELY(j.wide || s * j.lim[0] - c * j.lim[1] < 0.0f || j.lim[3] * c - j.lim[2] * s < 0.0f)) {
             float phi = f_atan2(s, c);
             if (phi < j.lo)
                 rot_block<PK, CK>(P, pk, C, ck, scl(a1, phi - j.lo));
@@ -586,7 +592,7 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Ancho
     V3 p = mad(c, n, -rad);
     float frac = 1.0f;
     V3 anc;
-    if (!anchored || dot(n, an.normal()) < 0.9f) {
+    if (ARC_UNLIKELY(!anchored || dot(n, an.normal()) < 0.9f)) {
         /* touch down: anchor where the contact point was at the substep start */
         V3 c0 = at_world0<AT>(pose0(), at);
         anc = mad(c0, n, -rad);
@@ -599,7 +605,7 @@ XD void friction(Body &b, const BodyK &k, V3 at, float rad, int &anchored, Ancho
     V3 e = sub(p, anc);
     e = mad(e, n, -dot(e, n));
     float l = len(e);
-    if (l < 1e-12f)
+    if (ARC_UNLIKELY(l < 1e-12f))
         return;
     V3 r = sub(p, b.x);
     float wt = wpos<KIND>(b, k, r, scl(e, recipf(l)));
@@ -622,7 +628,7 @@ XD void collide(Body &b, const BodyK &k, V3 at, float rad, int &touch, int &anch
     float ln = 0.0f, dmax = 0.0f;
     V3 c = at_world<AT>(b, at), nmax = v3(0.0f, 0.0f, 1.0f);
     int hit = 0;
-    if (!(fabsf(c.x) < 1e6f && fabsf(c.y) < 1e6f)) {
+    if (ARC_UNLIKELY(!(fabsf(c.x) < 1e6f && fabsf(c.y) < 1e6f))) {
         anchored = 0;
         return;
     }
