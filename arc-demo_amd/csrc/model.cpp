@@ -22,6 +22,8 @@ void terrain_flat(HostWorld &w, int nx, int ny, float cell, float height)
     w.t.cell = cell;
     w.t.ox = -0.5f * nx * cell;
     w.t.oy = -0.5f * ny * cell;
+    w.t.inv_cell = 1.0f / cell;
+    w.t.fast_cell = 0;
     w.outside = height;
     w.h.assign((size_t)nx * ny, height);
 }

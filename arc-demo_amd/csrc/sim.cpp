@@ -160,6 +160,19 @@ arc_sim *arc_sim_create(int n_worlds, double dt, int substeps, int grid, double 
         delete s;
         return nullptr;
     }
+    {
+        /* the spheres' column windows divide by the column size: a
+         * three-instruction quotient where the device check proves it exact
+         * and the numerators stay in its domain (radii and a non-zero grid
+         * origin in [2^-60, 2^90), see sphere_window) */
+        Model &Mq = s->su.model;
+        auto in_range = [](float v) { return fabsf(v) >= 0x1p-60f && fabsf(v) < 0x1p90f; };
+        bool ok = (Mq.terr.ox == 0.0f || in_range(Mq.terr.ox)) && (Mq.terr.oy == 0.0f || in_range(Mq.terr.oy));
+        for (int k = 0; k < NS; ++k)
+            ok = ok && in_range(Mq.sphere[k].rad);
+        Mq.terr.inv_cell = 1.0f / Mq.terr.cell;
+        Mq.terr.fast_cell = ok && quotient_checked(Mq.terr.cell);
+    }
     const Model &M = s->su.model;
     if (!build_wave_tables(s->su.tmpl, (float)cell, s->wave)) {
         fprintf(stderr, "libarcsim: world exceeds the wavefront kernel's tables\n");
@@ -706,6 +719,11 @@ int arcx_world_error(const arc_sim *cs, int i)
 }
 
 int arcx_last_error(const arc_sim *s) { return s->err; }
+int arcx_fast_divisors(const arc_sim *s)
+{
+    const Model &M = s->su.model;
+    return M.terr.fast_cell ? 1 : 0;
+}
 const char *arcx_error_message(const arc_sim *s) { return s->err ? s->msg : ""; }
 void arcx_clear_error(arc_sim *s) { s->err = 0; s->msg[0] = 0; }
 

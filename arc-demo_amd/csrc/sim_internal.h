@@ -11,6 +11,13 @@
 #include "setup.h"
 #include "wave_tables.h"
 
+namespace xpbd {
+/* whether qdiv's three-instruction quotient by d equals the IEEE quotient for
+ * every binary32 numerator of its domain on the current device: an exhaustive
+ * device check (selfcheck.hip), run once per device and divisor */
+bool quotient_checked(float d);
+} /* namespace xpbd */
+
 using namespace xpbd;
 
 struct BatchState;

@@ -74,6 +74,31 @@ XD float recipf(float x)
     return 1.0f / x;
 }
 
+/* a / d for a divisor fixed per simulation (the column size).  On the device,
+ * with y = RN(1 / d): q0 = a y, r = d q0 - a (one fused multiply-add),
+ * q = q0 - r y, three instructions instead of the ten of the general
+ * division.  `fast` is set only for a divisor for which this sequence has been
+ * checked, on the device, to give the IEEE quotient a / d bit for bit for
+ * every binary32 numerator that is zero or has 2^-96 <= |a| < 2^100 (all 2^32
+ * patterns, selfcheck.hip); the caller guarantees its numerators lie there.
+ * Unchecked divisors take the division. */
+struct Divisor {
+    float d, y;
+    int fast;
+};
+XD bool qdiv_domain(float a) { return a == 0.0f || (fabsf(a) >= 0x1p-96f && fabsf(a) < 0x1p100f); }
+XD float qdiv(float a, const Divisor &D)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    if (D.fast) {
+        const float q0 = a * D.y;
+        const float r = __builtin_fmaf(D.d, q0, -a);
+        return __builtin_fmaf(-r, D.y, q0);
+    }
+#endif
+    return a / D.d;
+}
+
 /* binary64 evaluation rounded to binary32 */
 XD float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 XD float f_sin(float x) { return (float)sin((double)x); }
@@ -542,6 +567,8 @@ struct TerrainRef {
 struct TerrainK {
     int nx, ny;
     float cell, ox, oy;
+    float inv_cell; /* RN(1 / cell) */
+    int fast_cell;  /* column indices through qdiv (checked exact, numerators in its domain) */
 };
 XD float dequant(const TerrainRef &r, int k) { return k == Q_OUT ? r.outside : 0.03f * (float)k - r.base; }
 XD float terrain_cell(const TerrainK &t, const TerrainRef &r, int ix, int iy)
@@ -559,12 +586,18 @@ XD float terrain_height(const TerrainK &t, const TerrainRef &r, float x, float y
     return dequant(r, r.q[(int)fx + (int)fy * r.pitch]);
 }
 
+/* the columns a sphere's box overlaps.  Callers have checked |c.x|, |c.y| <
+ * 1e6, and fast_cell is set only when every radius lies in [2^-60, 2^90) and
+ * ox, oy are zero or lie in the same range: then each numerator (c -+ rad) - o
+ * is zero or at least 2^-84 (a difference of floats of magnitude >= 2^-61 is a
+ * multiple of 2^-84) and below 2^100, inside qdiv's checked domain */
 XD void sphere_window(const TerrainK &t, V3 c, float rad, int &ix0, int &ix1, int &iy0, int &iy1)
 {
-    ix0 = (int)floorf((c.x - rad - t.ox) / t.cell);
-    ix1 = (int)floorf((c.x + rad - t.ox) / t.cell);
-    iy0 = (int)floorf((c.y - rad - t.oy) / t.cell);
-    iy1 = (int)floorf((c.y + rad - t.oy) / t.cell);
+    const Divisor D = { t.cell, t.inv_cell, t.fast_cell };
+    ix0 = (int)floorf(qdiv(c.x - rad - t.ox, D));
+    ix1 = (int)floorf(qdiv(c.x + rad - t.ox, D));
+    iy0 = (int)floorf(qdiv(c.y - rad - t.oy, D));
+    iy1 = (int)floorf(qdiv(c.y + rad - t.oy, D));
 }
 
 /* column heights straight from the world's grid */

@@ -31,6 +31,18 @@ int arcx_device_count(void);
  * or -1 on a HIP error. */
 int arcx_check_recip(unsigned long long *mismatches);
 
+/* Self-check of the three-instruction quotient a / d the step kernels use for
+ * a divisor fixed per sim (the column size) on the current GPU: every binary32
+ * numerator a that is zero or has 2^-96 <= |a| < 2^100, against the IEEE
+ * quotient; writes the number of differing results.  A sim enables the fast
+ * quotient for a divisor only when this count is 0.  Returns 0, or -1 on a
+ * HIP error. */
+int arcx_check_quotient(float d, unsigned long long *mismatches);
+
+/* Which of the sim's divisors take the checked fast quotient: bit 0 the
+ * column size (the spheres' column windows). */
+int arcx_fast_divisors(const arc_sim *s);
+
 /* n successive outputs of the engine's xorshift128 generator seeded `seed`
  * (the stream the reset draws from; also used to make synthetic inputs). */
 void arcx_rng_stream(unsigned int seed, long long n, unsigned int *out);

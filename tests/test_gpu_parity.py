@@ -60,6 +60,25 @@ def test_fast_reciprocal_exhaustive(native):
     assert bad.value == 0
 
 
+@pytest.mark.parametrize("grid,cell", [(64, 0.4), (32, 0.25), (48, 0.37)])
+def test_fast_column_quotient_checked(native, grid, cell):
+    """A sim divides by its column size with the three-instruction quotient
+    exactly when the exhaustive device check finds no numerator of the
+    quotient's domain (zero, 2^-96 <= |a| < 2^100) where it differs from the
+    IEEE quotient; for these column sizes the check passes, so the step kernels
+    take the fast path (and the parity tests cover it)."""
+    lib = native.sim_lib()
+    s = native.Sim(4, grid=grid, cell=cell)
+    try:
+        mask = lib.arcx_fast_divisors(s.ptr)
+    finally:
+        s.close()
+    bad = ctypes.c_ulonglong(12345)
+    assert lib.arcx_check_quotient(ctypes.c_float(cell), ctypes.addressof(bad)) == 0
+    assert bad.value == 0
+    assert mask == 1
+
+
 def test_template_matches_oracle(native):
     g = native.Sim(4)
     o = P.Lib("o32").sim(4)
