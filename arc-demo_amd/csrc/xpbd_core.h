@@ -87,14 +87,18 @@ struct Divisor {
     int fast;
 };
 XD bool qdiv_domain(float a) { return a == 0.0f || (fabsf(a) >= 0x1p-96f && fabsf(a) < 0x1p100f); }
+/* the three-instruction sequence itself (host builds evaluate it for tests) */
+XD float qdiv_fma(float a, float d, float y)
+{
+    const float q0 = a * y;
+    const float r = __builtin_fmaf(d, q0, -a);
+    return __builtin_fmaf(-r, y, q0);
+}
 XD float qdiv(float a, const Divisor &D)
 {
 #ifdef __HIP_DEVICE_COMPILE__
-    if (D.fast) {
-        const float q0 = a * D.y;
-        const float r = __builtin_fmaf(D.d, q0, -a);
-        return __builtin_fmaf(-r, D.y, q0);
-    }
+    if (D.fast)
+        return qdiv_fma(a, D.d, D.y);
 #endif
     return a / D.d;
 }

@@ -201,6 +201,28 @@ extern "C" void hc_joint_consts(const HC *h, int j, double *o)
 /* The reset's draws made as RESET_LANES jump-ahead chunks (the device's way) against
  * the one sequential stream, and the split footprint scan against the plain
  * one.  Returns the number of differing values (0 expected). */
+/* The column quotient's sequence (xpbd_core.h qdiv_fma, the device's fast
+ * path) on the host, whose fmaf is the same correctly rounded fused
+ * multiply-add, against the IEEE quotient a / d over the binary32 patterns
+ * lo, lo + step, ... below hi (NaN and infinities skipped; with in_domain, also
+ * the numerators outside qdiv's domain).  Returns the mismatches. */
+extern "C" unsigned long long hc_quotient_mismatches(float d, unsigned long long lo, unsigned long long hi,
+                                                     unsigned long long step, int in_domain)
+{
+    const float y = 1.0f / d;
+    unsigned long long bad = 0;
+    for (unsigned long long u = lo; u < hi; u += step) {
+        const uint32_t bits = (uint32_t)u;
+        float a;
+        memcpy(&a, &bits, 4);
+        if (!(fabsf(a) <= 3.4028235e38f) || (in_domain && !xpbd::qdiv_domain(a)))
+            continue;
+        const float f = xpbd::qdiv_fma(a, d, y), q = a / d;
+        bad += memcmp(&f, &q, 4) != 0 ? 1u : 0u;
+    }
+    return bad;
+}
+
 extern "C" int hc_reset_split_check(const HC *h, unsigned seed, int gen)
 {
     const ResetK &R = h->su.rk;

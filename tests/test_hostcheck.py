@@ -92,3 +92,26 @@ def test_reset_split_matches_sequential(grid, cell):
     for seed in (0, 1, 12345, 4294967295):
         for gen in (1, 0):
             assert h.L.hc_reset_split_check(h.p, seed, gen) == 0, (seed, gen)
+
+
+def test_column_quotient_sequence():
+    """The column windows' three-instruction quotient (q0 = a y, r = d q0 - a,
+    q = q0 - r y with y = RN(1/d), two fused multiply-adds), evaluated on the
+    host with the same correctly rounded FMA: equal to the IEEE quotient for
+    every sampled numerator of its domain (zero, 2^-96 <= |a| < 2^100), for
+    the column sizes the tests and the benchmark use; below 2^-96 the residual
+    underflows and the sequence does differ for some numerators, which is why a
+    sim enables it only where its numerators stay in the domain (the device
+    check then covers all 2^32 patterns)."""
+    import ctypes
+    L = ctypes.CDLL(str(HOSTCHECK))
+    f = L.hc_quotient_mismatches
+    f.restype = ctypes.c_ulonglong
+    f.argtypes = [ctypes.c_float, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_int]
+    for cell in (0.4, 0.25, 0.37):
+        assert f(cell, 0, 1 << 32, 1009, 1) == 0, cell
+        # every pattern of one binade in the middle of the domain, both signs
+        assert f(cell, 0x3F800000, 0x40000000, 1, 1) == 0
+        assert f(cell, 0xBF800000, 0xC0000000, 1, 1) == 0
+    # below the domain the sequence is not exact (tiny numerators of both signs)
+    assert f(0.4, 0, 0x0F800000, 3, 0) > 0
