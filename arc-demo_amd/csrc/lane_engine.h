@@ -47,6 +47,18 @@ XD size_t launder(size_t v)
     return v;
 }
 template <class T> XD T *launder(T *p) { return (T *)launder((size_t)p); }
+/* A uniform value as a scalar register pair of its own.  Values taken from the
+ * kernel's argument structure otherwise stay parts of the wide scalar tuples
+ * they were loaded in, and a tuple spilled under register pressure is
+ * reloaded whole (one v_readlane per register) wherever one member is used. */
+XD size_t own_sgpr(size_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(v));
+#endif
+    return v;
+}
+template <class T> XD T *own_sgpr(T *p) { return (T *)own_sgpr((size_t)p); }
 
 template <int I, int N, class F> XD void static_for(F &&f)
 {

@@ -371,18 +371,19 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
     }
     if (d.unstable[i])
         return; /* frozen until reset, csrc/arc_sim.c:135-136 */
-    GlobalAnchors A{ d.anc, d.ancn, S, (size_t)i };
+    const size_t So = own_sgpr(S);
+    GlobalAnchors A{ own_sgpr(d.anc), own_sgpr(d.ancn), So, (size_t)i };
     TerrainRef tr = terrain_ref(d, i);
     tr.base = d.tbase[i];
     tr.outside = d.outside[i];
     ContactStore cs;
-    cs.base = d.lcbuf;
-    cs.stride = P * S;
+    cs.base = own_sgpr(d.lcbuf);
+    cs.stride = P * So;
     cs.world = (size_t)i * P + q;
     cs.mirror = crec;
     cs.mstride = BLOCK;
-    cs.counts = d.ncontact;
-    cs.cstride = S;
+    cs.counts = own_sgpr(d.ncontact);
+    cs.cstride = So;
     cs.cworld = (size_t)i;
     constexpr int WPB = BLOCK / P;
     int ov;
