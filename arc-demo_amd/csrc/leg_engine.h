@@ -112,6 +112,55 @@ __device__ __forceinline__ JointK leg_joint(int q, const Model &M)
     return J;
 }
 
+/* Where a lane finds the constants of its legs' joints: selected from the
+ * compile-time constants of the four legs at each use (SelJoints), or, for the
+ * mid and distal joints, read from a table of the four legs' geometry in LDS
+ * (LdsJoints: [m - 1][leg][r1, r2, a1, a2, b1, b2], filled by fill_joint_table),
+ * which costs LDS reads instead of three selects per value.  Same values. */
+constexpr int JGEO = 18;
+struct SelJoints {
+    template <int M_> __device__ __forceinline__ JointK get(int leg, const Model &M) const { return leg_joint<M_>(leg, M); }
+};
+struct LdsJoints {
+    const float *g;
+    template <int M_> __device__ __forceinline__ JointK get(int leg, const Model &M) const
+    {
+        if constexpr (M_ == 0) {
+            return leg_joint<0>(leg, M);
+        } else {
+            JointK J = leg_joint<M_>(0, M); /* gains, limits and the drive's constants: every leg's */
+            const float *p = g + ((M_ - 1) * LEGS + leg) * JGEO;
+            J.r1 = v3(p[0], p[1], p[2]);
+            J.r2 = v3(p[3], p[4], p[5]);
+            J.a1 = v3(p[6], p[7], p[8]);
+            J.a2 = v3(p[9], p[10], p[11]);
+            J.b1 = v3(p[12], p[13], p[14]);
+            J.b2 = v3(p[15], p[16], p[17]);
+            return J;
+        }
+    }
+};
+/* thread 0 of the workgroup writes the table (2 x 4 x 18 values); the caller
+ * synchronises the workgroup before its first read */
+__device__ __forceinline__ void fill_joint_table(float *g)
+{
+    if (threadIdx.x != 0)
+        return;
+    static_for<1, 3>([&](auto m) {
+        static_for<0, LEGS>([&](auto l) {
+            constexpr JointK J = robot::joint[3 * l + m];
+            float *p = g + ((m - 1) * LEGS + l) * JGEO;
+            const V3 v[6] = { J.r1, J.r2, J.a1, J.a2, J.b1, J.b2 };
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                p[3 * k] = v[k].x;
+                p[3 * k + 1] = v[k].y;
+                p[3 * k + 2] = v[k].z;
+            }
+        });
+    });
+}
+
 /* State of one world in one lane of its group: the torso replica, the
  * lane's K legs (bodies 0 coxa, 1 femur, 2 tibia) and their joints; spheres
  * in lane-local order 0 torso, then 1 + 5 l + m for sphere m of local leg l. */
@@ -339,9 +388,10 @@ template <int WPB> struct RootsShared {
  * Anchors: global sphere indices; Poses: local bodies 0 torso, 1 + 3 l + m;
  * the contact store addresses this lane's own list.  Returns this lane's
  * contact-overflow bits (bit per global contact body, as the lane form). */
-template <int K, class Anchors, class Poses, class Tile, class Roots>
+template <int K, class Anchors, class Poses, class Tile, class Roots, class JG = SelJoints>
 __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Poses &P0, const Model &M,
-                                        const TerrainRef &tr0, const ContactStore &cs0, const Tile &T0, Roots &RJ)
+                                        const TerrainRef &tr0, const ContactStore &cs0, const Tile &T0, Roots &RJ,
+                                        const JG &jg = JG())
 {
     constexpr int P = LEGS / K;
     int overflow = 0;
@@ -376,12 +426,12 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
         RJ.positions(W, q, M);
         static_for<0, K>([&](auto l) {
             {
-                const JointK J = leg_joint<1>(q * K + l, M);
+                const JointK J = jg.template get<1>(q * K + l, M);
                 joint_positions<KISO, KANI>(&W.b[l][0], &robot::body[1], W.b[l][1], robot::body[2], J,
                                             W.target[3 * l + 1], M.hh, W.tacc[3 * l + 1]);
             }
             {
-                const JointK J = leg_joint<2>(q * K + l, M);
+                const JointK J = jg.template get<2>(q * K + l, M);
                 joint_positions<KANI, KANI>(&W.b[l][1], &robot::body[2], W.b[l][2], robot::body[3], J,
                                             W.target[3 * l + 2], M.hh, W.tacc[3 * l + 2]);
             }
@@ -441,12 +491,12 @@ __device__ __forceinline__ int leg_step(LegWorld<K> &W, int q0, Anchors &A0, Pos
         RJ.velocities(W, q, M);
         static_for<0, K>([&](auto l) {
             {
-                const JointK J = leg_joint<1>(q * K + l, M);
+                const JointK J = jg.template get<1>(q * K + l, M);
                 joint_velocities<KISO, KANI>(&W.b[l][0], &robot::body[1], W.b[l][1], robot::body[2], J, M.h,
                                              W.tacc[3 * l + 1]);
             }
             {
-                const JointK J = leg_joint<2>(q * K + l, M);
+                const JointK J = jg.template get<2>(q * K + l, M);
                 joint_velocities<KANI, KANI>(&W.b[l][1], &robot::body[2], W.b[l][2], robot::body[3], J, M.h,
                                              W.tacc[3 * l + 2]);
             }

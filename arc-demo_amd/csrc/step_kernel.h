@@ -358,7 +358,8 @@ template <int K> using LegRoots = std::conditional_t<K == 1, RootsPair, RootsLoc
 /* one control step of world i by lane q of its group */
 template <int BLOCK, int TILE, int K>
 __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &M, const float *__restrict__ row,
-                                               LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q, float *crec)
+                                               LdsPoses<BLOCK> &P0, int8_t *s_tile, int i, int q, float *crec,
+                                               const float *s_jgeo)
 {
     constexpr int P = LEGS / K;
     const size_t S = d.stride;
@@ -393,11 +394,11 @@ __device__ __forceinline__ void step_world_leg(const DevWorlds &d, const Model &
                                                              d.x[0 * S + i], d.x[1 * S + i]);
         load_leg<K>(d, i, q, W);
         LegRoots<K> roots;
-        ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile, roots);
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, tile, roots, LdsJoints{ s_jgeo });
     } else {
         load_leg<K>(d, i, q, W);
         LegRoots<K> roots;
-        ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile(), roots);
+        ov = leg_step<K>(W, q, A, P0, M, tr, cs, GridTile(), roots, LdsJoints{ s_jgeo });
     }
     store_leg<K>(d, opaque_index((size_t)i), q, W, ov);
 }
@@ -411,6 +412,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
     __shared__ float s_pose[(1 + 3 * K) * 7 * BLOCK];
     __shared__ int8_t s_tile[TILE > 0 ? TILE * TILE * (BLOCK / P) : 1];
     __shared__ float s_crec[K == 1 ? CMIRROR * 6 * BLOCK : 1]; /* each lane's first tibia contact records */
+    __shared__ float s_jgeo[2 * LEGS * JGEO];                  /* the legs' mid and distal joint geometry */
+    fill_joint_table(s_jgeo);
+    __syncthreads();
     const Model &M = *m;
     LdsPoses<BLOCK> P0;
     P0.p = s_pose;
@@ -437,7 +441,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(K == 1 ? 
             const int i = batch ? batch_world(batch, n_ranges, w, row) : w;
             if (!batch)
                 row = targets + (size_t)w * NJ;
-            step_world_leg<BLOCK, TILE, K>(d, M, row, P0, s_tile, i, t % P, K == 1 ? s_crec + threadIdx.x : nullptr);
+            step_world_leg<BLOCK, TILE, K>(d, M, row, P0, s_tile, i, t % P, K == 1 ? s_crec + threadIdx.x : nullptr,
+                                           s_jgeo);
         }
         base = next(base);
     }
